@@ -1,0 +1,177 @@
+"""Problem definitions shared by the fixture generator and the tests.
+
+Each case is plain data (numbers, uint8 maps, kind codes), so the same case
+can be fed to the upstream solver (fixture generation, this container only),
+to the C oracle and to the HIP path.  Boundary kinds are codes, turned into
+modules by whichever package runs the case:
+
+  BND_CONDUCTING=0, BND_ANISO=1, BND_INSULATING=2, BND_THIN=3 (param h)
+
+Materials: (fixed, kz, ky, kx, rho, c, is_tensor).  is_tensor=1 means k is
+given as a diagonal 3x3 matrix in (z,y,x) order (for the anisotropic
+boundary); 0 means a scalar k (kz=ky=kx).
+
+Volumetric: tuples in the upstream layout, with the z-decay entry's z grid
+given as the string "zgrid" (replaced by the cell-centre z array).
+"""
+import numpy as np
+
+BND_CONDUCTING, BND_ANISO, BND_INSULATING, BND_THIN = 0, 1, 2, 3
+
+NO_SOURCE, IMPULSE_SOURCE, STEPPED_SOURCE, IMPULSE_POINT_SOURCE_JOULES, \
+    SPATIALLY_Z_DECAYING_TEMPORAL_IMPULSE = 0, 1, 2, 3, 4
+
+
+def _zero(nz, ny, nx):
+    return (np.zeros((nz, ny, nx), np.uint8), np.zeros((nz + 1, ny, nx), np.uint8),
+            np.zeros((nz, ny + 1, nx), np.uint8), np.zeros((nz, ny, nx + 1), np.uint8),
+            np.zeros((nz, ny, nx), np.uint8))
+
+
+def _outer(bz, by, bx, cls):
+    bx[:, :, 0] = cls
+    bx[:, :, -1] = cls
+    by[:, 0, :] = cls
+    by[:, -1, :] = cls
+    bz[0] = cls
+    bz[-1] = cls
+
+
+def case_tiny_mixed():
+    """Three materials (two conducting + FIXED), conducting / insulating /
+    thin-layer faces, impulse + stepped + point sources."""
+    nz, ny, nx = 4, 5, 6
+    d = (4e-4, 5e-4, 6e-4)
+    dt = 2e-3
+    m, bz, by, bx, v = _zero(nz, ny, nx)
+    _outer(bz, by, bx, 1)
+    m[2:, :, :] = 1            # lower layers: second material
+    m[3, 0:2, 0:2] = 2         # FIXED corner block
+    m[1, 4, 5] = 2             # isolated FIXED cell touching the outer faces
+    bz[2, 1:4, 1:5] = 2        # thin layer patch between layers 1 and 2
+    bx[0, 2, 3] = 2            # one thin x-face
+    by[3, 2, 1:3] = 2          # thin y-faces
+    v[0] = 1
+    v[1, 2, 2] = 2
+    v[2, 1:3, 3:5] = 3
+    materials = [(0, 50.0, 50.0, 50.0, 7850.0, 470.0, 0),
+                 (0, 0.5, 0.5, 0.5, 1200.0, 1500.0, 0),
+                 (1, 0, 0, 0, 0, 0, 0)]
+    boundaries = [(BND_CONDUCTING, 0.0), (BND_INSULATING, 0.0), (BND_THIN, 2000.0)]
+    volumetric = [(NO_SOURCE,), (IMPULSE_SOURCE, 0.0, 1e4 / d[0]),
+                  (IMPULSE_POINT_SOURCE_JOULES, 3 * dt, 0.02),
+                  (STEPPED_SOURCE, 2 * dt, 5 * dt, 3e7)]
+    T0 = 293.0 + 0.01 * np.arange(nz * ny * nx, dtype=np.float64).reshape(nz, ny, nx)
+    T0[m == 2] = 300.0
+    return dict(name="tiny_mixed", shape=(nz, ny, nx), d=d, dt=dt, t0=0.0, nsteps=12,
+                materials=materials, boundaries=boundaries, volumetric=volumetric,
+                maps=(m, bz, by, bx, v), T0=T0, matrices=True)
+
+
+def case_tiny_aniso():
+    """Diagonal-tensor k with the anisotropic face law, thin layers, FIXED
+    cells, a scalar-k material on isotropic faces elsewhere is not mixed in
+    (the upstream rejects scalar k on anisotropic faces)."""
+    nz, ny, nx = 5, 6, 4
+    d = (3e-4, 5e-4, 4e-4)
+    dt = 1e-3
+    m, bz, by, bx, v = _zero(nz, ny, nx)
+    bz[:], by[:], bx[:] = 0, 0, 0
+    _outer(bz, by, bx, 1)
+    m[3:, :, :] = 1
+    m[4, 5, 0:2] = 2
+    bz[2, 2:5, 1:3] = 2
+    bx[1, 1:4, 2] = 2
+    v[0] = 1
+    materials = [(0, 0.6, 5.0, 3.0, 1550.0, 1200.0, 1),
+                 (0, 20.0, 15.0, 25.0, 2700.0, 900.0, 1),
+                 (1, 0, 0, 0, 0, 0, 0)]
+    boundaries = [(BND_ANISO, 0.0), (BND_INSULATING, 0.0), (BND_THIN, 500.0)]
+    volumetric = [(NO_SOURCE,),
+                  (SPATIALLY_Z_DECAYING_TEMPORAL_IMPULSE, 0.0, (1.0, 0.0, 0.0), 0.0,
+                   "zgrid", d[0], 1e4, 4e-4)]
+    T0 = np.full((nz, ny, nx), 293.0)
+    T0[m == 2] = 280.0
+    return dict(name="tiny_aniso", shape=(nz, ny, nx), d=d, dt=dt, t0=0.0, nsteps=10,
+                materials=materials, boundaries=boundaries, volumetric=volumetric,
+                maps=(m, bz, by, bx, v), T0=T0, matrices=True)
+
+
+def case_steel_cube(n=40, nsteps=200):
+    """40x40x40 steel cube, insulating faces, 1e4 J/m^2 flash into the top
+    z-layer at t=0, dt=1 ms, 200 steps (the CPU reference workload)."""
+    nz = ny = nx = n
+    d = (5e-4, 5e-4, 5e-4)
+    dt = 1e-3
+    m, bz, by, bx, v = _zero(nz, ny, nx)
+    _outer(bz, by, bx, 1)
+    v[0] = 1
+    materials = [(0, 50.0, 50.0, 50.0, 7850.0, 470.0, 0)]
+    boundaries = [(BND_CONDUCTING, 0.0), (BND_INSULATING, 0.0)]
+    volumetric = [(NO_SOURCE,), (IMPULSE_SOURCE, 0.0, 1e4 / d[0])]
+    T0 = np.full((nz, ny, nx), 293.0)
+    return dict(name="steel_cube%d" % n, shape=(nz, ny, nx), d=d, dt=dt, t0=0.0,
+                nsteps=nsteps, materials=materials, boundaries=boundaries,
+                volumetric=volumetric, maps=(m, bz, by, bx, v), T0=T0, matrices=False)
+
+
+def case_two_layer(nz=256, ny=256, nx=256, nsteps=1000, steel_layers=32, seed=1,
+                   h=250.0, dxyz=5e-4, dt=5e-4):
+    """Steel layer (first `steel_layers` z-cells) over foam, thin insulating
+    layer between them, insulating sides, flash 1e4*(1+0.05*U(-1,1)) J/m^2."""
+    rng = np.random.default_rng(seed)
+    E = 1e4 * (1.0 + 0.05 * rng.uniform(-1.0, 1.0))
+    d = (dxyz, dxyz, dxyz)
+    m, bz, by, bx, v = _zero(nz, ny, nx)
+    _outer(bz, by, bx, 1)
+    m[steel_layers:] = 1
+    bz[steel_layers] = 2
+    v[0] = 1
+    materials = [(0, 50.0, 50.0, 50.0, 7850.0, 470.0, 0),
+                 (0, 0.03, 0.03, 0.03, 30.0, 1500.0, 0)]
+    boundaries = [(BND_CONDUCTING, 0.0), (BND_INSULATING, 0.0), (BND_THIN, h)]
+    volumetric = [(NO_SOURCE,), (IMPULSE_SOURCE, 0.0, E / d[0])]
+    T0 = np.full((nz, ny, nx), 293.0)
+    return dict(name="two_layer_%dx%dx%d" % (nz, ny, nx), shape=(nz, ny, nx), d=d, dt=dt,
+                t0=0.0, nsteps=nsteps, materials=materials, boundaries=boundaries,
+                volumetric=volumetric, maps=(m, bz, by, bx, v), T0=T0, matrices=False,
+                flash_J_m2=E)
+
+
+def case_laminate(nz=128, ny=512, nx=512, nsteps=2000, seed=2, h=100.0, dxyz=5e-4,
+                  dt=2e-3, patch_min=10, patch_max=80):
+    """Anisotropic laminate kz=0.6, ky=kx=5, rho=1550, c=1200 on anisotropic
+    faces, 1..8 rectangular delaminations as thin layers at random depths,
+    flash 1e4 J/m^2 on top."""
+    rng = np.random.default_rng(seed)
+    d = (dxyz, dxyz, dxyz)
+    m, bz, by, bx, v = _zero(nz, ny, nx)
+    _outer(bz, by, bx, 1)
+    npatch = int(rng.integers(1, 9))
+    for _ in range(npatch):
+        sy = int(rng.integers(patch_min, min(patch_max, ny) + 1))
+        sx = int(rng.integers(patch_min, min(patch_max, nx) + 1))
+        y0 = int(rng.integers(0, ny - sy + 1))
+        x0 = int(rng.integers(0, nx - sx + 1))
+        kz = int(rng.integers(1, nz))
+        bz[kz, y0:y0 + sy, x0:x0 + sx] = 2
+    v[0] = 1
+    materials = [(0, 0.6, 5.0, 5.0, 1550.0, 1200.0, 1)]
+    boundaries = [(BND_ANISO, 0.0), (BND_INSULATING, 0.0), (BND_THIN, h)]
+    volumetric = [(NO_SOURCE,), (IMPULSE_SOURCE, 0.0, 1e4 / d[0])]
+    T0 = np.full((nz, ny, nx), 293.0)
+    return dict(name="laminate_%dx%dx%d" % (nz, ny, nx), shape=(nz, ny, nx), d=d, dt=dt,
+                t0=0.0, nsteps=nsteps, materials=materials, boundaries=boundaries,
+                volumetric=volumetric, maps=(m, bz, by, bx, v), T0=T0, matrices=False,
+                npatch=npatch)
+
+
+def zgrid_for(case):
+    nz, ny, nx = case["shape"]
+    dz = case["d"][0]
+    z = (np.arange(nz) + 0.5) * dz
+    return np.broadcast_to(z[:, None, None], (nz, ny, nx)).copy()
+
+
+def source_times(case):
+    return [case["t0"] + n * case["dt"] for n in range(case["nsteps"])]
