@@ -1,0 +1,151 @@
+"""Benchmark: FP64 ADI cell-steps/s of the plain step at 256^3 on one GPU.
+
+Workload: the two-layer stack -- 256x256x256 cells of 0.5 mm, steel
+(k=50, rho=7850, c=470) in the first 32 z-layers over foam (k=0.03, rho=30,
+c=1500), a thin insulating layer (h=250 W m^-2 K^-1) between them,
+insulating sides, a flash of 1e4*(1+0.05 U(-1,1)) J/m^2 (seed 1) into the
+top layer at t=0, dt=0.5 ms, FP64, surface readout captured every step.
+Synthetic data of that shape; the uint8 maps and coefficient tables are
+built by heatsim2_amd.setup exactly as a user would.
+
+A step is one full Douglas ADI step (x, y, z substeps) over all cells.
+`value` = cells x K / wall time of K steps run by hs2_run with the field and
+maps resident in HBM, bracketed by torch.cuda.synchronize().
+
+roofline: the dominant sweep kernel's algorithmic bytes per launch (x: 16 B
+per cell, y and z: 24 B per cell -- the FP64 field passes of the delta-form
+step, 64 B per cell-step in all) over its mean launch time from HIP events
+recorded on the launch stream around every substep kernel of K further steps
+(hs2_run_timed); peak = 8 TB/s.
+
+cpu_baseline: the oracle's reference-structured step (assembled permuted
+CSR operators, global Thomas with precomputed LU, transposes -- the
+upstream's structure) on one host thread, over a few steps of the same
+256^3 workload, reported as cell-steps/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = (16.0, 24.0, 24.0)  # x, y, z sweep algorithmic bytes
+METRIC = "FP64 ADI cell-steps/s @256³ + batched fwd+adjoint gradients/s vs plain"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(case, nsteps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    t0 = time.time()
+    coef = oracle.assemble_case(case)
+    plan = oracle.RefPlan(case["shape"], coef)
+    t_setup = time.time() - t0
+    T = np.array(case["T0"])
+    n = int(np.prod(case["shape"]))
+    t0 = time.time()
+    for _ in range(nsteps):
+        T = plan.step(T)
+    dt = time.time() - t0
+    del plan
+    return dict(value=n * nsteps / dt, unit="cell-steps/s", cores=1, kind="port",
+                sample="%d source-free steps of the same %dx%dx%d two-layer workload through the "
+                       "oracle's reference-structured step (permuted CSR B/C operators, global "
+                       "Thomas with precomputed LU, transposes), one thread; %.1f s timed, "
+                       "assembly %.1f s untimed" % ((nsteps,) + tuple(case["shape"]) + (dt, t_setup)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--size", type=int, nargs=3, default=(256, 256, 256))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import heatsim2_amd as hs
+    from heatsim2_amd import _lib
+    from hs2_cases import cases, to_package_args
+
+    nz, ny, nx = args.size
+    case = cases.case_two_layer(nz=nz, ny=ny, nx=nx, nsteps=1000)
+    pa = to_package_args(hs, case)
+    t0 = time.time()
+    P, S = hs.setup(*pa["setup"])
+    log("setup %.2f s" % (time.time() - t0))
+    dev = _lib.device()
+    n = nz * ny * nx
+    dt = case["dt"]
+    T = torch.from_numpy(case["T0"]).to(dev)
+    # warmup: the flash step plus W-1 more
+    T, _ = hs.run_adi_steps_many(P, 0.0, dt, max(args.warmup, 1), T, pa["vmap"], pa["volumetric"])
+    K = args.steps
+    hist = torch.empty((K, ny, nx), dtype=torch.float64, device=dev)
+    L = _lib.load()
+    import ctypes
+    stream = _lib.stream_handle()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check("hs2_run", L.hs2_run(P.plan, _lib.ptr(T), K, None, None, 1,
+                                    ctypes.c_void_p(hist.data_ptr()), stream))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = n * K / wall
+    log("timed %d steps: %.3f s, %.3f ms/step, %.3e cell-steps/s" % (K, wall, 1e3 * wall / K, value))
+    assert torch.isfinite(T).all().item() and torch.isfinite(hist).all().item()
+
+    # per-kernel device time with HIP events on the launch stream
+    kms = (ctypes.c_double * 3)()
+    _lib.check("hs2_run_timed", L.hs2_run_timed(P.plan, _lib.ptr(T), K, kms, stream))
+    torch.cuda.synchronize()
+    kms = list(kms)
+    names = ("sweep_x", "sweep_y", "sweep_z")
+    gbs = [BYTES_PER_CELL[i] * n / (kms[i] * 1e-3) / 1e9 for i in range(3)]
+    dom = int(np.argmax(kms))
+    for i in range(3):
+        log("  %s: %.4f ms/launch, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
+            % (names[i], kms[i], gbs[i], 100 * gbs[i] / HBM_PEAK_GBS))
+    step_gbs = 64.0 * value / 1e9
+    roofline = dict(bound="hbm", kernel=names[dom], achieved=round(gbs[dom], 1),
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs[dom] / HBM_PEAK_GBS, 4),
+                    traffic=None, bytes_per_launch=BYTES_PER_CELL[dom] * n,
+                    kernel_ms=[round(x, 5) for x in kms],
+                    step_achieved=round(step_gbs, 1), step_frac=round(step_gbs / HBM_PEAK_GBS, 4))
+
+    out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": K,
+           "warmup": args.warmup, "ms_per_step": 1e3 * wall / K, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "two-layer steel/foam stack %dx%dx%d (z,y,x), dx=0.5 mm, thin "
+                                  "layer h=250 W/m^2K, flash %.1f J/m^2, dt=0.5 ms, FP64, surface "
+                                  "readout every step" % (nz, ny, nx, case["flash_J_m2"]),
+                      "cells": n, "parallelism": "single GPU"},
+           "roofline": roofline}
+    del T, hist
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+        log("cpu baseline %.3e cell-steps/s" % out["cpu_baseline"]["value"])
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,110 @@
+"""ctypes binding of libhs2.so, the HIP library (include/heatsim2_amd.h).
+
+There is no fallback: if the library is missing or no GPU is visible, every
+entry point raises.  Device buffers are torch tensors; their data_ptr() and
+torch's current HIP stream are handed to the C ABI.
+"""
+import ctypes
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhs2.so")
+
+HS2_OK = 0
+_ERRNAMES = {1: "HS2_E_ARG", 2: "HS2_E_UNSUPPORTED", 3: "HS2_E_HIP", 4: "HS2_E_STATE",
+             5: "HS2_E_ALLOC"}
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("nz", ctypes.c_int32), ("ny", ctypes.c_int32), ("nx", ctypes.c_int32),
+                ("dz", ctypes.c_double), ("dy", ctypes.c_double), ("dx", ctypes.c_double),
+                ("dt", ctypes.c_double)]
+
+
+class HS2Error(RuntimeError):
+    def __init__(self, func, code, msg):
+        super().__init__("%s failed (%s): %s" % (func, _ERRNAMES.get(code, code), msg))
+        self.code = code
+
+
+# every symbol include/heatsim2_amd.h declares
+EXPORTS = ("hs2_plan_create", "hs2_plan_destroy", "hs2_plan_set_compact", "hs2_plan_expand",
+           "hs2_step", "hs2_run", "hs2_run_timed", "hs2_surface_temperature", "hs2_last_error",
+           "hs2_version")
+
+_lib = None
+
+
+def load():
+    """Load libhs2.so (no GPU needed to load it)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("heatsim2_amd: HIP library %s is not built; run "
+                          "`python -c 'import __graft_entry__ as g; g.build()'`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.hs2_plan_create.argtypes = [ctypes.POINTER(GridDesc), ctypes.POINTER(vp)]
+    L.hs2_plan_create.restype = i32
+    L.hs2_plan_destroy.argtypes = [vp]
+    L.hs2_plan_destroy.restype = None
+    L.hs2_plan_set_compact.argtypes = [vp, vp, vp, vp, vp, i32, i32, vp, vp, vp]
+    L.hs2_plan_set_compact.restype = i32
+    L.hs2_plan_expand.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.hs2_plan_expand.restype = i32
+    L.hs2_step.argtypes = [vp, vp, vp, vp, vp, i32, vp, vp, vp]
+    L.hs2_step.restype = i32
+    L.hs2_run.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp]
+    L.hs2_run.restype = i32
+    L.hs2_run_timed.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_double), vp]
+    L.hs2_run_timed.restype = i32
+    L.hs2_surface_temperature.argtypes = [vp, i32, i32, i32, dbl, vp, vp]
+    L.hs2_surface_temperature.restype = i32
+    L.hs2_last_error.argtypes = []
+    L.hs2_last_error.restype = ctypes.c_char_p
+    L.hs2_version.argtypes = []
+    L.hs2_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(func, code):
+    if code != HS2_OK:
+        raise HS2Error(func, code, load().hs2_last_error().decode())
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("heatsim2_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    """data_ptr of a contiguous device tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("expected a contiguous device tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def host_ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def surface_temperature_device(T, dz):
+    if T.dim() != 3 or T.dtype != torch.float64:
+        raise ValueError("T must be a (nz, ny, nx) float64 tensor")
+    T = T.contiguous()
+    nz, ny, nx = T.shape
+    out = torch.empty((ny, nx), dtype=torch.float64, device=T.device)
+    check("hs2_surface_temperature",
+          load().hs2_surface_temperature(ptr(T), nz, ny, nx, float(dz), ptr(out), stream_handle()))
+    return out

@@ -1,0 +1,163 @@
+"""The time step: run_adi_steps and a device-resident multi-step driver.
+
+run_adi_steps keeps the upstream signature and semantics
+(alternatingdirection_c_pyx.pyx:287-416): the volumetric sources are
+evaluated for call time t (:294-386; an impulse fires only when t equals
+its time exactly), one full Douglas step x -> y -> z is taken, and a new
+array is returned (Tarray is not modified).  Device tensors stay on the
+device; numpy arrays are copied in and out.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from heatsim2_amd import _lib
+from heatsim2_amd.constants import (IMPULSE_POINT_SOURCE_JOULES, IMPULSE_SOURCE,
+                                    SPATIALLY_Z_DECAYING_TEMPORAL_IMPULSE, STEPPED_SOURCE)
+
+
+def source_tables(ADI_params, t, dt, volumetric, volumetric_elements):
+    """Per-class source values at call time t and, for z-decaying impulses,
+    a per-cell field.  Returns (vtab or None, sfield ndarray or None).
+    Follows alternatingdirection_c_pyx.pyx:294-386."""
+    nv = len(volumetric)
+    if nv > 256:
+        raise ValueError("at most 256 volumetric classes (uint8 map)")
+    vtab = np.zeros(max(nv, 1))
+    active = False
+    sfield = None
+    for idx, entry in enumerate(volumetric):
+        kind = entry[0]
+        if kind == IMPULSE_SOURCE:
+            if t == entry[1]:
+                vtab[idx] = entry[2] / dt
+                active = True
+        elif kind == STEPPED_SOURCE:
+            (_, starttime, endtime, wattsperm3) = entry
+            if t >= starttime and t <= endtime:
+                vtab[idx] = wattsperm3
+                active = True
+        elif kind == IMPULSE_POINT_SOURCE_JOULES:
+            if t == entry[1]:
+                vtab[idx] = entry[2] / (ADI_params.volume_array * dt)
+                active = True
+        elif kind == SPATIALLY_Z_DECAYING_TEMPORAL_IMPULSE:
+            (_, impulse_time, decaydirec, offset, z_ndgrid, ddz, joulesperm2,
+             characlength) = entry
+            if t == impulse_time:
+                decaydirec = np.asarray(decaydirec, dtype=float)
+                assert ((decaydirec == np.array([1.0, 0.0, 0.0])).all() or
+                        (decaydirec == np.array([-1.0, 0.0, 0.0])).all())
+                offset = float(offset)
+                ddz = abs(float(ddz))
+                joulesperm2 = float(joulesperm2)
+                characlength = float(characlength)
+                assert characlength > 0
+                sel = volumetric_elements == idx
+                centre = np.asarray(z_ndgrid, dtype=float) * decaydirec[0] - offset
+                right = centre + ddz / 2.0
+                left = np.maximum(centre - ddz / 2.0, 0.0)
+                use = sel & (right > 0.0)
+                if sfield is None:
+                    sfield = np.zeros(ADI_params.shape)
+                zint = -np.exp(-right[use] / characlength) + np.exp(-left[use] / characlength)
+                sfield[use] = zint * joulesperm2 / (ddz * dt)
+    return (vtab if active else None), sfield
+
+
+def _device_vmap(ADI_params, volumetric_elements):
+    cached = getattr(ADI_params, "_vmap_host", None)
+    if cached is None or cached.shape != volumetric_elements.shape or \
+            not np.array_equal(cached, volumetric_elements):
+        if volumetric_elements.dtype != np.uint8:
+            raise ValueError("volumetric_elements must be uint8")
+        ADI_params._vmap_host = np.array(volumetric_elements, copy=True)
+        ADI_params._vmap_dev = torch.from_numpy(ADI_params._vmap_host).to(_lib.device())
+    return ADI_params._vmap_dev
+
+
+def _as_device_field(Tarray, shape):
+    dev = _lib.device()
+    if isinstance(Tarray, torch.Tensor):
+        if Tarray.dtype != torch.float64:
+            raise ValueError("Tarray must be float64")
+        T = Tarray.to(dev).contiguous()
+        return T, True
+    a = np.ascontiguousarray(Tarray, dtype=np.float64)
+    if a.shape != tuple(shape):
+        raise ValueError("Tarray shape %s does not match grid %s" % (a.shape, tuple(shape)))
+    return torch.from_numpy(a).to(dev), False
+
+
+def run_adi_steps(ADI_params, ADI_steps, t, dt, Tarray, volumetric_elements, volumetric):
+    """One ADI time step from call time t; returns the new temperature field
+    (same kind as Tarray: numpy array or device tensor)."""
+    if len(ADI_steps) != 3:
+        raise ValueError("expected the three substeps returned by setup()")
+    T, on_dev = _as_device_field(Tarray, ADI_params.shape)
+    if tuple(T.shape) != tuple(ADI_params.shape):
+        raise ValueError("Tarray shape %s does not match grid %s" % (tuple(T.shape), ADI_params.shape))
+    vtab, sfield = source_tables(ADI_params, t, dt, volumetric, volumetric_elements)
+    out = torch.empty_like(T)
+    vmap = _device_vmap(ADI_params, volumetric_elements) if vtab is not None else None
+    sf = torch.from_numpy(sfield).to(T.device) if sfield is not None else None
+    L = _lib.load()
+    _lib.check("hs2_step", L.hs2_step(ADI_params.plan, _lib.ptr(T), _lib.ptr(out), _lib.ptr(vmap),
+                                      _lib.host_ptr(vtab), 0 if vtab is None else len(vtab),
+                                      _lib.ptr(sf), None, _lib.stream_handle()))
+    if on_dev:
+        return out
+    return out.cpu().numpy()
+
+
+def run_adi_steps_many(ADI_params, t0, dt, nsteps, Tarray, volumetric_elements, volumetric,
+                       surface_history=False):
+    """nsteps calls of run_adi_steps at t = t0 + n*dt, kept on the device.
+
+    Returns (T_final, surface_history) where surface_history is the
+    (nsteps, ny, nx) insulating-top readout after every step (or None).
+    Steps whose sources are all per-class constants run through one
+    hs2_run call; a step that fires a z-decaying impulse runs on its own.
+    """
+    T, on_dev = _as_device_field(Tarray, ADI_params.shape)
+    T = T.clone()
+    nz, ny, nx = ADI_params.shape
+    hist = torch.empty((nsteps, ny, nx), dtype=torch.float64, device=T.device) \
+        if surface_history else None
+    nv = max(len(volumetric), 1)
+    rows = np.zeros((nsteps, nv))
+    special = {}
+    for n in range(nsteps):
+        vtab, sfield = source_tables(ADI_params, t0 + n * dt, dt, volumetric, volumetric_elements)
+        if sfield is not None:
+            special[n] = (vtab, sfield)
+        elif vtab is not None:
+            rows[n, :len(vtab)] = vtab
+    L = _lib.load()
+    vmap = _device_vmap(ADI_params, volumetric_elements) if rows.any() or special else None
+    stream = _lib.stream_handle()
+    n = 0
+    while n < nsteps:
+        stop = min([s for s in special if s >= n] + [nsteps])
+        if stop > n:
+            seg = np.ascontiguousarray(rows[n:stop])
+            hp = None if hist is None else ctypes.c_void_p(hist[n:stop].data_ptr())
+            _lib.check("hs2_run", L.hs2_run(ADI_params.plan, _lib.ptr(T), stop - n,
+                                            _lib.ptr(vmap) if seg.any() else None,
+                                            _lib.host_ptr(seg) if seg.any() else None, nv, hp,
+                                            stream))
+            n = stop
+        if n < nsteps and n in special:
+            vtab, sfield = special[n]
+            sf = torch.from_numpy(sfield).to(T.device)
+            hp = None if hist is None else ctypes.c_void_p(hist[n].data_ptr())
+            _lib.check("hs2_step", L.hs2_step(ADI_params.plan, _lib.ptr(T), _lib.ptr(T),
+                                              _lib.ptr(vmap) if vtab is not None else None,
+                                              _lib.host_ptr(vtab),
+                                              0 if vtab is None else len(vtab),
+                                              _lib.ptr(sf), hp, stream))
+            n += 1
+    if on_dev:
+        return T, hist
+    return T.cpu().numpy(), (None if hist is None else hist.cpu().numpy())

@@ -1,0 +1,303 @@
+"""setup(): build a device plan for the ADI Crank-Nicolson step.
+
+Drop-in for heatsim2's crank_nicolson.setup (crank_nicolson.pyx:128-498),
+same signature and argument conventions.  Where the upstream derives each
+cell's equation symbolically and accumulates assembled matrices through C
+(crank_nicolson.pyx:272-491, alternatingdirection_c.c:102-199), this
+version:
+
+1. evaluates every boundary plugin's face law once per (face class, axis,
+   material pair) that occurs in the maps, applying the upstream's
+   neighbour-k rule -- a neighbour's k counts only if it is in range and
+   COMPUTE, else the cell's own k is used (crank_nicolson.pyx:298-338) --
+   and reduces it to one conductance per volume g = -alpha/d for a flux
+   q = alpha (T_p - T_m) (crank_nicolson.pyx:359-363);
+2. M = rho*c/dt per COMPUTE material (crank_nicolson.pyx:378), M = 1 and
+   no coupling for FIXED (crank_nicolson.pyx:473-486);
+3. hands the uint8 maps (kept on the device) and these small tables to the
+   HIP library, whose sweep kernels derive every coefficient on the fly.
+
+Validation happens here, before any launch, with the upstream's failure
+modes turned into exceptions: IndexError for map entries beyond the tables,
+TypeError / ValueError for a conductivity type that does not fit the face
+law (expression.py:311-313, :737), ValueError for flux through an outer face
+of a COMPUTE cell (the upstream exits, alternatingdirection_c.c:158-163).
+"""
+import numbers
+
+import numpy as np
+import torch
+
+from heatsim2_amd import _lib, expression
+from heatsim2_amd.constants import TEMPERATURE_COMPUTE, TEMPERATURE_FIXED
+
+# placeholder names of each face law's temperature arguments; the first two
+# are the minus / plus cells across the face (crank_nicolson.pyx:220-250)
+_T_ARGS = (
+    ("Tm55", "Tp55", "Tm45", "Tp45", "Tm65", "Tp65", "Tm54", "Tp54", "Tm56", "Tp56", "Tm46",
+     "Tp46", "Tm64", "Tp64"),
+    ("T5m5", "T5p5", "T4m5", "T4p5", "T6m5", "T6p5", "T5m4", "T5p4", "T5m6", "T5p6", "T4m6",
+     "T4p6", "T6m4", "T6p4"),
+    ("T55m", "T55p", "T45m", "T45p", "T65m", "T65p", "T54m", "T54p", "T56m", "T56p", "T46m",
+     "T46p", "T64m", "T64p"),
+)
+_AXIS_NAME = ("z", "y", "x")
+# compact tables must fit the kernels' LDS budget (doubles)
+MAX_TABLE_DOUBLES = 4096
+
+
+class adi_params(object):
+    """Plan for one grid (alternatingdirection_c_pyx.pyx:75-85 fields plus
+    the device state)."""
+    shape = None
+    volume_array = None
+
+    def __init__(self, **kwargs):
+        for key, val in kwargs.items():
+            setattr(self, key, val)
+        self._plan = None
+
+    @property
+    def plan(self):
+        if self._plan is None:
+            raise RuntimeError("plan released")
+        return self._plan
+
+    def release(self):
+        if getattr(self, "_plan", None) is not None:
+            _lib.load().hs2_plan_destroy(self._plan)
+            self._plan = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class adi_step(object):
+    """Descriptor of one Douglas substep (alternatingdirection_c_pyx.pyx:
+    120-167): the axis it solves and the upstream's permutation.  The
+    operators themselves live on the device."""
+
+    def __init__(self, ADI_params, permuteorder, stepnum):
+        self.ADI_params = ADI_params
+        self.permuteorder = tuple(permuteorder)
+        self.stepnum = stepnum
+        self.permutedshape = [ADI_params.shape[i] for i in permuteorder]
+        inv = sorted((p, i) for i, p in enumerate(permuteorder))
+        self.invpermuteorder = tuple(i for _, i in inv)
+        self.axis = ("x", "y", "z")[stepnum]
+
+
+def _material_table(materials, dt):
+    fixed, kvals, M = [], [], []
+    for idx, mat in enumerate(materials):
+        if mat[0] == TEMPERATURE_COMPUTE:
+            (_, k, rho, c) = mat
+            fixed.append(False)
+            kvals.append(k)
+            M.append(rho * c * (1.0 / dt))
+        elif mat[0] == TEMPERATURE_FIXED:
+            fixed.append(True)
+            kvals.append(None)
+            M.append(1.0)
+        else:
+            raise AssertionError("material %d: first member must be TEMPERATURE_COMPUTE or "
+                                 "TEMPERATURE_FIXED" % idx)
+    return fixed, kvals, M
+
+
+class _FaceLaws(object):
+    """Face conductance per volume for (class, axis, k_minus, k_plus)."""
+
+    def __init__(self, boundaries, d):
+        self.boundaries = boundaries
+        self.d = d  # (dz, dy, dx)
+        self._expr = {}
+        self._cache = {}
+
+    def expr(self, b, axis):
+        key = (b, axis)
+        if key not in self._expr:
+            bnd = self.boundaries[b]
+            plugin, extra = bnd[0], tuple(bnd[1:])
+            fn = (plugin.qz, plugin.qy, plugin.qx)[axis]
+            KM = expression.linear_expression("kmat_m")
+            KP = expression.linear_expression("kmat_p")
+            Ts = [expression.linear_expression(n) for n in _T_ARGS[axis]]
+            dz, dy, dx = self.d
+            self._expr[key] = fn(KM, KP, dz, dy, dx, *Ts, *extra)
+        return self._expr[key]
+
+    def g(self, b, axis, km, kp, kkey):
+        key = (b, axis, kkey)
+        if key in self._cache:
+            return self._cache[key]
+        where = "boundary %d (%s faces)" % (b, _AXIS_NAME[axis])
+        const, terms = expression.linear_form(self.expr(b, axis), {"kmat_m": km, "kmat_p": kp})
+        if const != 0.0:
+            raise ValueError("%s: the flux has a constant term %g; only fluxes linear in the "
+                             "temperatures are supported (the upstream asserts on it)"
+                             % (where, const))
+        tm, tp = _T_ARGS[axis][0], _T_ARGS[axis][1]
+        alpha = beta = 0.0
+        for (name, gid), coef in terms.items():
+            if gid is None:
+                raise ValueError("%s: the term in %s lies outside group(); such terms are "
+                                 "integrated explicitly, which this build does not support"
+                                 % (where, name))
+            if name == tp:
+                alpha += coef
+            elif name == tm:
+                beta += coef
+            else:
+                raise ValueError("%s: the flux depends on %s (off-diagonal conductivity); that "
+                                 "needs unaligned_anisotropic mode, which is not supported"
+                                 % (where, name))
+        if abs(alpha + beta) > 1e-12 * max(abs(alpha), abs(beta)):
+            raise ValueError("%s: flux %g*T_plus + %g*T_minus is not a two-point difference"
+                             % (where, alpha, beta))
+        g = -alpha / self.d[axis]
+        if g < 0.0:
+            raise ValueError("%s: negative face conductance %g" % (where, g))
+        self._cache[key] = g
+        return g
+
+
+def _check_map(name, arr, shape):
+    if not isinstance(arr, np.ndarray) or arr.dtype != np.uint8 or arr.ndim != 3:
+        raise ValueError("Buffer dtype mismatch: %s must be a 3-D uint8 ndarray" % name)
+    if tuple(arr.shape) != tuple(shape):
+        raise ValueError("%s has shape %s, expected %s" % (name, arr.shape, tuple(shape)))
+    return np.ascontiguousarray(arr)
+
+
+def _pair_counts(b, mm, mp, nmat):
+    key = (b.astype(np.int32) * nmat + mm) * nmat + mp
+    return np.bincount(key.ravel(), minlength=1)
+
+
+def build_tables(d, dt, materials, boundaries, mat, bz, by, bx):
+    """Compact tables (gtab[nb,3,nmat,nmat], mtab[nmat], compute[nmat]) for
+    the maps, evaluated only for the (class, axis, material pair) that occur.
+    Raises like the upstream setup on invalid input."""
+    nmat, nb = len(materials), len(boundaries)
+    if nmat == 0 or nb == 0:
+        raise ValueError("need at least one material and one boundary class")
+    if nmat > 256 or nb > 256:
+        raise ValueError("at most 256 materials / boundary classes (uint8 maps)")
+    for name, a, lim in (("material_elements", mat, nmat), ("boundary_z_elements", bz, nb),
+                         ("boundary_y_elements", by, nb), ("boundary_x_elements", bx, nb)):
+        if a.size and int(a.max()) >= lim:
+            raise IndexError("%s holds index %d but only %d entries are defined"
+                             % (name, int(a.max()), lim))
+    fixed, kvals, M = _material_table(materials, dt)
+    laws = _FaceLaws(boundaries, d)
+    gtab = np.zeros((nb, 3, nmat, nmat))
+    used = np.bincount(mat.ravel(), minlength=nmat) > 0
+
+    def kk(m):
+        return kvals[m]
+
+    faces = (
+        (0, bz[1:-1], mat[:-1], mat[1:], (bz[0], mat[0]), (bz[-1], mat[-1])),
+        (1, by[:, 1:-1], mat[:, :-1], mat[:, 1:], (by[:, 0], mat[:, 0]), (by[:, -1], mat[:, -1])),
+        (2, bx[:, :, 1:-1], mat[:, :, :-1], mat[:, :, 1:], (bx[:, :, 0], mat[:, :, 0]),
+         (bx[:, :, -1], mat[:, :, -1])),
+    )
+    for axis, bint, mm, mp, outer0, outer1 in faces:
+        if bint.size:
+            counts = _pair_counts(bint, mm, mp, nmat)
+            for key in np.nonzero(counts)[0]:
+                b, rem = divmod(int(key), nmat * nmat)
+                ma, mb = divmod(rem, nmat)
+                if fixed[ma] and fixed[mb]:
+                    continue
+                km_i = ma if not fixed[ma] else mb
+                kp_i = mb if not fixed[mb] else ma
+                gtab[b, axis, ma, mb] = laws.g(b, axis, kk(km_i), kk(kp_i), (km_i, kp_i))
+        for bo, mo in (outer0, outer1):
+            counts = np.bincount((bo.astype(np.int32) * nmat + mo).ravel(), minlength=1)
+            for key in np.nonzero(counts)[0]:
+                b, m = divmod(int(key), nmat)
+                if fixed[m]:
+                    continue
+                if laws.g(b, axis, kk(m), kk(m), (m, m)) != 0.0:
+                    raise ValueError(
+                        "Equation exceeds bounds of domain: boundary class %d on an outer %s "
+                        "face of a COMPUTE cell carries flux. Are external boundaries set "
+                        "correctly? (outer faces must be insulating)" % (b, _AXIS_NAME[axis]))
+    mtab = np.array([M[m] if (used[m] or fixed[m]) else 1.0 for m in range(nmat)], dtype=np.float64)
+    for m in range(nmat):
+        if used[m] and not fixed[m] and not (mtab[m] > 0.0 and np.isfinite(mtab[m])):
+            raise ValueError("material %d: rho*c/dt = %g must be positive" % (m, mtab[m]))
+    compute = np.array([0 if f else 1 for f in fixed], dtype=np.uint8)
+    return gtab, mtab, compute
+
+
+def setup(z0, y0, x0,
+          dz, dy, dx,
+          nz, ny, nx,
+          dt,
+          materials,
+          boundaries,
+          volumetric,
+          material_elements,
+          boundary_z_elements,
+          boundary_y_elements,
+          boundary_x_elements,
+          volumetric_elements,
+          top_surface_y_curvatures=None,
+          top_surface_x_curvatures=None,
+          unaligned_anisotropic=False):
+    """Prepare the ADI step for a grid; returns (ADI_params, ADI_steps).
+
+    Same arguments as the upstream setup (crank_nicolson.pyx:128-142).
+    z0, y0, x0 are accepted and unused, as upstream.  The curvature and
+    unaligned-anisotropic modes are not provided.
+    """
+    if top_surface_y_curvatures is not None or top_surface_x_curvatures is not None:
+        raise NotImplementedError("top-surface curvature mode is not provided by heatsim2_amd")
+    if unaligned_anisotropic:
+        raise NotImplementedError("unaligned_anisotropic (off-diagonal K) mode is not provided "
+                                  "by heatsim2_amd")
+    for v in (nz, ny, nx):
+        if not isinstance(v, (numbers.Integral, np.integer)) or v < 1:
+            raise ValueError("nz, ny, nx must be positive integers")
+    nz, ny, nx = int(nz), int(ny), int(nx)
+    mat = _check_map("material_elements", material_elements, (nz, ny, nx))
+    bz = _check_map("boundary_z_elements", boundary_z_elements, (nz + 1, ny, nx))
+    by = _check_map("boundary_y_elements", boundary_y_elements, (nz, ny + 1, nx))
+    bx = _check_map("boundary_x_elements", boundary_x_elements, (nz, ny, nx + 1))
+    _check_map("volumetric_elements", volumetric_elements, (nz, ny, nx))
+    d = (float(dz), float(dy), float(dx))
+    gtab, mtab, compute = build_tables(d, float(dt), materials, boundaries, mat, bz, by, bx)
+    nmat, nb = len(materials), len(boundaries)
+    if gtab.size + 2 * nmat > MAX_TABLE_DOUBLES:
+        raise NotImplementedError(
+            "%d materials x %d boundary classes exceed the compact coefficient table (%d "
+            "doubles)" % (nmat, nb, MAX_TABLE_DOUBLES))
+    L = _lib.load()
+    dev = _lib.device()
+    params = adi_params(shape=(nz, ny, nx), volume_array=dz * dy * dx, d=d, dt=float(dt),
+                        nmat=nmat, nb=nb, gtab=gtab, mtab=mtab, compute=compute)
+    params.d_mat = torch.from_numpy(mat).to(dev)
+    params.d_bz = torch.from_numpy(bz).to(dev)
+    params.d_by = torch.from_numpy(by).to(dev)
+    params.d_bx = torch.from_numpy(bx).to(dev)
+    import ctypes
+    grid = _lib.GridDesc(nz, ny, nx, d[0], d[1], d[2], float(dt))
+    h = ctypes.c_void_p()
+    _lib.check("hs2_plan_create", L.hs2_plan_create(ctypes.byref(grid), ctypes.byref(h)))
+    params._plan = h
+    gt = np.ascontiguousarray(gtab)
+    _lib.check("hs2_plan_set_compact",
+               L.hs2_plan_set_compact(h, _lib.ptr(params.d_mat), _lib.ptr(params.d_bz),
+                                      _lib.ptr(params.d_by), _lib.ptr(params.d_bx), nmat, nb,
+                                      _lib.host_ptr(gt), _lib.host_ptr(mtab),
+                                      _lib.host_ptr(compute)))
+    torch.cuda.synchronize()
+    steps = [adi_step(params, (0, 1, 2), 0), adi_step(params, (2, 0, 1), 1),
+             adi_step(params, (1, 2, 0), 2)]
+    return params, steps

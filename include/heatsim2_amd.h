@@ -1,0 +1,125 @@
+/* heatsim2_amd: C ABI of the MI355X (gfx950) 3-D ADI Crank-Nicolson step.
+ *
+ * This is the boundary the Python package heatsim2_amd binds (ctypes), and
+ * that any other host would bind.  Conventions:
+ *   - every function returns an int status (HS2_OK = 0) and never exits or
+ *     aborts; hs2_last_error() gives a message for the calling thread;
+ *   - every large buffer is caller-owned device memory passed as a plain
+ *     pointer (fields are (nz,ny,nx) C-order float64; maps are uint8 in the
+ *     upstream layout); the plan owns only its small tables and one n-double
+ *     workspace;
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
+ *   - a plan is used from one stream at a time; distinct plans are
+ *     independent.
+ *
+ * Upstream interface each entry point replaces is cited as file:line of the
+ * heatsim2 source tree.
+ */
+#ifndef HEATSIM2_AMD_H
+#define HEATSIM2_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  HS2_OK = 0,
+  HS2_E_ARG = 1,          /* bad argument (null pointer, size, index)           */
+  HS2_E_UNSUPPORTED = 2,  /* valid but outside what the kernels handle         */
+  HS2_E_HIP = 3,          /* a HIP runtime call failed                         */
+  HS2_E_STATE = 4,        /* plan not ready (coefficients not set)             */
+  HS2_E_ALLOC = 5         /* device or host allocation failed                  */
+};
+
+typedef struct hs2_plan hs2_plan;
+
+typedef struct {
+  int32_t nz, ny, nx;     /* cells along z (slowest), y, x (contiguous)       */
+  double dz, dy, dx;      /* spacing, m                                        */
+  double dt;              /* time step, s                                      */
+} hs2_grid_desc;
+
+/* Create a plan for one grid.
+ * Replaces adi_setup + pyadi_step.__init__ / create_adi_step
+ * (alternatingdirection_c_pyx.pyx:418-445, :137-167;
+ *  alternatingdirection_c.h:51). */
+int hs2_plan_create(const hs2_grid_desc *grid, hs2_plan **out);
+
+/* Release a plan.  Replaces delete_adi_step (alternatingdirection_c.h:49,
+ * alternatingdirection_c.c:14-54) and pyadi_step.__del__
+ * (alternatingdirection_c_pyx.pyx:169-175). */
+void hs2_plan_destroy(hs2_plan *plan);
+
+/* Compact coefficients: the upstream uint8 maps stay on the device and the
+ * sweeps derive every coefficient on the fly from three small tables.
+ *   d_mat (nz,ny,nx), d_bz (nz+1,ny,nx), d_by (nz,ny+1,nx), d_bx (nz,ny,nx+1):
+ *       device uint8 maps, caller-owned, must outlive the plan's use;
+ *   h_gtab[((b*3 + axis)*nmat + m_minus)*nmat + m_plus], axis 0=z 1=y 2=x:
+ *       face conductance per volume (W m^-3 K^-1) of a class-b face between
+ *       a minus-side cell of material m_minus and a plus-side cell of
+ *       material m_plus, as seen by the COMPUTE cell(s) beside it (the
+ *       neighbour-k fallback already applied);
+ *   h_mtab[nmat]: rho*c/dt for COMPUTE materials, 1 for FIXED;
+ *   h_compute[nmat]: 1 COMPUTE, 0 FIXED.
+ * Outer faces carry no flux (validated by the caller, as the upstream
+ * requires).  Replaces the per-cell add_equation / finalize assembly
+ * (crank_nicolson.pyx:272-496; alternatingdirection_c.h:53;
+ *  alternatingdirection_c_pyx.pyx:177-282). */
+int hs2_plan_set_compact(hs2_plan *plan, const uint8_t *d_mat, const uint8_t *d_bz,
+                         const uint8_t *d_by, const uint8_t *d_bx, int nmat, int nb,
+                         const double *h_gtab, const double *h_mtab, const uint8_t *h_compute);
+
+/* Expand the plan's compact coefficients into per-face float64 arrays on the
+ * device (d_gz (nz+1,ny,nx), d_gy (nz,ny+1,nx), d_gx (nz,ny,nx+1), d_M (n),
+ * d_cflag (n, uint8)); any pointer may be NULL.  Exposes the numbers the
+ * upstream keeps in Amat/Bmat (alternatingdirection_c_pyx.pyx:230,251) for
+ * inspection. */
+int hs2_plan_expand(hs2_plan *plan, double *d_gz, double *d_gy, double *d_gx, double *d_M,
+                    uint8_t *d_cflag, void *stream);
+
+/* One full Douglas step T^n -> T^{n+1} (x, y, z substeps).
+ *   d_Tin, d_Tout: n doubles (may alias: in-place is supported);
+ *   source (optional, the volumetric power density at this step's time):
+ *     S_c = vtab[vmap[c]] + sfield[c] for COMPUTE cells;
+ *     d_vmap: uint8 class map or NULL; h_vtab: nvtab host doubles (nvtab<=256);
+ *     d_sfield: n doubles or NULL;
+ *   d_surf: optional (ny,nx) output of the insulating-top surface readout of
+ *     T^{n+1}, fused into the z substep.
+ * Replaces run_adi_steps (alternatingdirection_c_pyx.pyx:287-416). */
+int hs2_step(hs2_plan *plan, const double *d_Tin, double *d_Tout, const uint8_t *d_vmap,
+             const double *h_vtab, int nvtab, const double *d_sfield, double *d_surf,
+             void *stream);
+
+/* nsteps steps in place on d_T.  h_vtab_steps (nsteps x nvtab, host) gives
+ * each step's per-class source values, or NULL for no source; steps whose
+ * row is all zero run without reading the class map.  d_surf_hist
+ * (nsteps x ny x nx) receives the readout after every step, or NULL.
+ * Replaces the user-side loop over run_adi_steps
+ * (demos/testheatsim2.py:105-110). */
+int hs2_run(hs2_plan *plan, double *d_T, int nsteps, const uint8_t *d_vmap,
+            const double *h_vtab_steps, int nvtab, double *d_surf_hist, void *stream);
+
+/* nsteps source-free steps in place on d_T, like hs2_run, with HIP events
+ * recorded on `stream` around every substep kernel; kernel_ms[0..2] receive
+ * the mean device time per launch of the x, y and z sweep.  A measurement
+ * entry point with no upstream counterpart (the upstream has no timers). */
+int hs2_run_timed(hs2_plan *plan, double *d_T, int nsteps, double *kernel_ms, void *stream);
+
+/* Insulating-top surface readout of a (nz,ny,nx) field into (ny,nx):
+ * T0 - 5/16 (T1 - T0).  Replaces insulating_z_min_surface_temperature
+ * (surface_temperature.py:4-36). */
+int hs2_surface_temperature(const double *d_T, int nz, int ny, int nx, double dz,
+                            double *d_out, void *stream);
+
+/* Message for the last failed call on this thread ("" if none). */
+const char *hs2_last_error(void);
+
+/* Library version string. */
+const char *hs2_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

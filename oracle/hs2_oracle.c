@@ -141,11 +141,6 @@ struct orc_refplan {
   double *tv, *rhs, *res[3], *tmp;
 };
 
-typedef struct {
-  int64_t row, col;
-  double v;
-} coo_t;
-
 /* permuted row index of cell (k,j,i): alternatingdirection_c.c:114 */
 static size_t prow(const orc_sub *s, const int64_t pos[3]) {
   return ((size_t)s->pshape[1] * (size_t)pos[s->perm[0]] + (size_t)pos[s->perm[1]]) *
@@ -153,32 +148,66 @@ static size_t prow(const orc_sub *s, const int64_t pos[3]) {
          (size_t)pos[s->perm[2]];
 }
 
-static int coo_cmp(const void *a, const void *b) {
-  const coo_t *x = a, *y = b;
-  if (x->row != y->row) return x->row < y->row ? -1 : 1;
-  if (x->col != y->col) return x->col < y->col ? -1 : 1;
-  return 0;
+/* Rows with at most `cap` entries each, filled in any order, then sorted by
+ * column with duplicates summed -- what coo_matrix(...).tocsr() gives
+ * (alternatingdirection_c_pyx.pyx:251,269). */
+typedef struct {
+  int64_t *cols;
+  double *vals;
+  uint8_t *cnt;
+  int cap;
+} rowbuf_t;
+
+static int rowbuf_init(rowbuf_t *b, size_t n, int cap) {
+  b->cap = cap;
+  b->cols = malloc(sizeof(int64_t) * n * (size_t)cap);
+  b->vals = malloc(sizeof(double) * n * (size_t)cap);
+  b->cnt = calloc(n, 1);
+  return (b->cols && b->vals && b->cnt) ? ORC_OK : ORC_E_ALLOC;
 }
 
-/* coo -> csr with duplicates summed (alternatingdirection_c_pyx.pyx:251,269) */
-static int coo_to_csr(coo_t *e, int64_t ne, size_t n, csr_t *out) {
-  qsort(e, (size_t)ne, sizeof(coo_t), coo_cmp);
-  out->indptr = calloc(n + 1, sizeof(int64_t));
-  out->indices = malloc(sizeof(int64_t) * (size_t)(ne ? ne : 1));
-  out->data = malloc(sizeof(double) * (size_t)(ne ? ne : 1));
+static void rowbuf_free(rowbuf_t *b) {
+  free(b->cols); free(b->vals); free(b->cnt);
+}
+
+static void rowbuf_add(rowbuf_t *b, size_t r, int64_t col, double v) {
+  size_t at = r * (size_t)b->cap + b->cnt[r]++;
+  b->cols[at] = col;
+  b->vals[at] = v;
+}
+
+static int rowbuf_to_csr(rowbuf_t *b, size_t n, csr_t *out) {
+  out->indptr = malloc(sizeof(int64_t) * (n + 1));
+  size_t total = 0;
+  for (size_t r = 0; r < n; r++) total += b->cnt[r];
+  out->indices = malloc(sizeof(int64_t) * (total ? total : 1));
+  out->data = malloc(sizeof(double) * (total ? total : 1));
   if (!out->indptr || !out->indices || !out->data) return ORC_E_ALLOC;
   int64_t w = 0;
-  for (int64_t t = 0; t < ne; t++) {
-    if (w > 0 && out->indices[w - 1] == e[t].col && t > 0 && e[t - 1].row == e[t].row) {
-      out->data[w - 1] += e[t].v;
-    } else {
-      out->indices[w] = e[t].col;
-      out->data[w] = e[t].v;
-      out->indptr[e[t].row + 1]++;
-      w++;
+  out->indptr[0] = 0;
+  for (size_t r = 0; r < n; r++) {
+    int64_t *c = b->cols + r * (size_t)b->cap;
+    double *v = b->vals + r * (size_t)b->cap;
+    int m = b->cnt[r];
+    for (int x = 1; x < m; x++) { /* insertion sort, stable */
+      int64_t cc = c[x];
+      double vv = v[x];
+      int y = x - 1;
+      while (y >= 0 && c[y] > cc) { c[y + 1] = c[y]; v[y + 1] = v[y]; y--; }
+      c[y + 1] = cc;
+      v[y + 1] = vv;
     }
+    for (int x = 0; x < m; x++) {
+      if (x > 0 && c[x] == c[x - 1]) {
+        out->data[w - 1] += v[x];
+      } else {
+        out->indices[w] = c[x];
+        out->data[w] = v[x];
+        w++;
+      }
+    }
+    out->indptr[r + 1] = w;
   }
-  for (size_t r = 0; r < n; r++) out->indptr[r + 1] += out->indptr[r];
   out->nnz = w;
   return ORC_OK;
 }
@@ -249,12 +278,12 @@ orc_refplan *orc_ref_create(int nz, int ny, int nx, const double *coef) {
     S->L = malloc(3 * n * sizeof(double));
     S->U = malloc(3 * n * sizeof(double));
     S->D = calloc(n, sizeof(double));
-    coo_t *eB = malloc(sizeof(coo_t) * 12 * n);
-    coo_t *eC[2] = {NULL, NULL};
-    int64_t nB = 0, nCe[2] = {0, 0};
-    for (int c = 0; c < s; c++) eC[c] = malloc(sizeof(coo_t) * 4 * n);
-    if (!S->A || !S->L || !S->U || !S->D || !eB || (s > 0 && !eC[0]) || (s > 1 && !eC[1])) {
-      free(eB); free(eC[0]); free(eC[1]);
+    rowbuf_t eB, eC[2];
+    int rc = rowbuf_init(&eB, n, 10);
+    for (int c = 0; c < s && rc == ORC_OK; c++) rc = rowbuf_init(&eC[c], n, 4);
+    if (rc != ORC_OK || !S->A || !S->L || !S->U || !S->D) {
+      rowbuf_free(&eB);
+      for (int c = 0; c < s; c++) rowbuf_free(&eC[c]);
       orc_ref_destroy(p);
       return NULL;
     }
@@ -266,7 +295,7 @@ orc_refplan *orc_ref_create(int nz, int ny, int nx, const double *coef) {
           size_t r = prow(S, pos);
           if (q[1] == 0.0) { /* FIXED: A diag 1, B self +1 (crank_nicolson.pyx:481) */
             S->A[3 * r + 1] = 1.0;
-            eB[nB++] = (coo_t){(int64_t)r, (int64_t)r, 1.0};
+            rowbuf_add(&eB, r, (int64_t)r, 1.0);
             continue;
           }
           double self_m = q[0], self_bare = 0.0;
@@ -281,29 +310,30 @@ orc_refplan *orc_ref_create(int nz, int ny, int nx, const double *coef) {
               np_[ax] += side ? 1 : -1;
               size_t col = prow(S, np_);
               if (u < s) {
-                eC[u][nCe[u]++] = (coo_t){(int64_t)r, (int64_t)col, 0.5 * g};
-                eC[u][nCe[u]++] = (coo_t){(int64_t)r, (int64_t)r, -0.5 * g};
-                eB[nB++] = (coo_t){(int64_t)r, (int64_t)col, 0.5 * g};
+                rowbuf_add(&eC[u], r, (int64_t)col, 0.5 * g);
+                rowbuf_add(&eC[u], r, (int64_t)r, -0.5 * g);
+                rowbuf_add(&eB, r, (int64_t)col, 0.5 * g);
                 self_m -= 0.5 * g;
               } else if (u == s) {
                 S->A[3 * r + (side ? 2 : 0)] = -0.5 * g;
                 adiag += 0.5 * g;
-                eB[nB++] = (coo_t){(int64_t)r, (int64_t)col, 0.5 * g};
+                rowbuf_add(&eB, r, (int64_t)col, 0.5 * g);
                 self_m -= 0.5 * g;
               } else {
-                eB[nB++] = (coo_t){(int64_t)r, (int64_t)col, g};
+                rowbuf_add(&eB, r, (int64_t)col, g);
                 self_bare -= g;
               }
             }
           }
           S->A[3 * r + 1] = adiag;
-          eB[nB++] = (coo_t){(int64_t)r, (int64_t)r, self_m};
-          if (self_bare != 0.0) eB[nB++] = (coo_t){(int64_t)r, (int64_t)r, self_bare};
+          rowbuf_add(&eB, r, (int64_t)r, self_m);
+          if (self_bare != 0.0) rowbuf_add(&eB, r, (int64_t)r, self_bare);
           S->D[r] = q[1];
         }
-    int rc = coo_to_csr(eB, nB, n, &S->B);
-    for (int c = 0; c < s && rc == ORC_OK; c++) rc = coo_to_csr(eC[c], nCe[c], n, &S->C[c]);
-    free(eB); free(eC[0]); free(eC[1]);
+    rc = rowbuf_to_csr(&eB, n, &S->B);
+    for (int c = 0; c < s && rc == ORC_OK; c++) rc = rowbuf_to_csr(&eC[c], n, &S->C[c]);
+    rowbuf_free(&eB);
+    for (int c = 0; c < s; c++) rowbuf_free(&eC[c]);
     if (rc) { orc_ref_destroy(p); return NULL; }
     tridiaglu(n, S->A, S->L, S->U);
   }

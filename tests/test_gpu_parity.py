@@ -1,0 +1,200 @@
+"""HIP path against the upstream fixtures and the oracle (GPU)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from hs2_cases import cases, load_golden, to_package_args
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10  # max |T_gpu - T_ref| / peak temperature rise
+
+
+@pytest.fixture(scope="module")
+def hs():
+    import heatsim2_amd
+    return heatsim2_amd
+
+
+def _rel(a, b, rise):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b)))) / rise
+
+
+@pytest.mark.parametrize("name", ["tiny_mixed", "tiny_aniso"])
+@pytest.mark.parametrize("on_device", [False, True])
+def test_small_cases_match_upstream(hs, name, on_device):
+    case = load_golden(name)
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    ref = case["fixture"]["traj"]
+    rise = np.abs(ref - case["T0"]).max()
+    T = torch.from_numpy(case["T0"]).cuda() if on_device else case["T0"].copy()
+    T_in0 = T.clone() if on_device else T.copy()
+    for n in range(case["nsteps"]):
+        t = case["t0"] + n * case["dt"]
+        T_new = hs.run_adi_steps(P, S, t, case["dt"], T, args["vmap"], args["volumetric"])
+        if n == 0:  # the input is never modified
+            assert (torch.equal(T, T_in0) if on_device else np.array_equal(T, T_in0))
+        T = T_new
+        got = T.cpu().numpy() if on_device else T
+        assert _rel(got, ref[n], rise) < TOL, n
+    assert isinstance(T, torch.Tensor) == on_device
+
+
+@pytest.mark.parametrize("name", ["steel_cube40", "two_layer_mini", "laminate_mini"])
+def test_trajectories_match_upstream(hs, name):
+    case = load_golden(name)
+    z = case["fixture"]
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    st = int(z["top_stride"])
+    T, hist = hs.run_adi_steps_many(P, case["t0"], case["dt"], case["nsteps"],
+                                    torch.from_numpy(case["T0"]).cuda(), args["vmap"],
+                                    args["volumetric"], surface_history=True)
+    rise = np.abs(z["top_history"] - case["T0"][0, ::st, ::st]).max()
+    assert _rel(T.cpu().numpy(), z["final"], rise) < TOL
+    # readout history equals the readout of the upstream top layers
+    if "zprofile" in z.files:
+        pass
+    T_host = T.cpu().numpy()
+    assert np.allclose(hist[-1].cpu().numpy(), oracle.surface_readout(T_host, case["d"][0]),
+                       rtol=0, atol=1e-12 * rise)
+
+
+def test_steel_cube_history_matches_upstream(hs):
+    """The 40^3 steel flash case, step by step: the top-layer history and the
+    z profile at the centre over all 200 steps."""
+    case = load_golden("steel_cube40")
+    z = case["fixture"]
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    T = torch.from_numpy(case["T0"]).cuda()
+    st = int(z["top_stride"])
+    rise = np.abs(z["top_history"] - 293.0).max()
+    worst = 0.0
+    for n in range(case["nsteps"]):
+        T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"], args["volumetric"])
+        Th = T.cpu().numpy()
+        worst = max(worst, _rel(Th[0, ::st, ::st], z["top_history"][n], rise),
+                    _rel(Th[:, 20, 20], z["zprofile"][n], rise))
+    assert worst < TOL
+    # first-frame energy check: the flash lands in the top layer (E/(rho c dz))
+    assert abs(rise - 1e4 / (7850 * 470 * 5e-4)) / rise < 0.5
+
+
+def test_expand_matches_oracle_coefficients(hs):
+    """The device-expanded per-face conductances and M equal the oracle's
+    per-cell restatement of crank_nicolson.pyx."""
+    from heatsim2_amd import _lib
+    for name in ("tiny_mixed", "tiny_aniso", "laminate_mini"):
+        case = load_golden(name)
+        args = to_package_args(hs, case)
+        P, S = hs.setup(*args["setup"])
+        nz, ny, nx = case["shape"]
+        dev = torch.device("cuda")
+        gz = torch.zeros((nz + 1, ny, nx), dtype=torch.float64, device=dev)
+        gy = torch.zeros((nz, ny + 1, nx), dtype=torch.float64, device=dev)
+        gx = torch.zeros((nz, ny, nx + 1), dtype=torch.float64, device=dev)
+        M = torch.zeros((nz, ny, nx), dtype=torch.float64, device=dev)
+        cf = torch.zeros((nz, ny, nx), dtype=torch.uint8, device=dev)
+        _lib.check("hs2_plan_expand", _lib.load().hs2_plan_expand(
+            P.plan, _lib.ptr(gz), _lib.ptr(gy), _lib.ptr(gx), _lib.ptr(M), _lib.ptr(cf),
+            _lib.stream_handle()))
+        gz, gy, gx, M, cf = [t.cpu().numpy() for t in (gz, gy, gx, M, cf)]
+        ref = oracle.assemble_case(case).reshape(nz, ny, nx, 8)
+        c = cf.astype(float)
+        got = np.stack([M, c, gz[:-1] * c, gz[1:] * c, gy[:, :-1] * c, gy[:, 1:] * c,
+                        gx[:, :, :-1] * c, gx[:, :, 1:] * c], axis=-1)
+        assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)) < 1e-15, name
+
+
+def _random_case(shape, seed):
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape
+    m = rng.integers(0, 3, size=shape).astype(np.uint8)
+    m[rng.random(shape) < 0.05] = 3  # FIXED
+    bz = rng.integers(0, 2, size=(nz + 1, ny, nx)).astype(np.uint8) * 2
+    by = rng.integers(0, 2, size=(nz, ny + 1, nx)).astype(np.uint8) * 2
+    bx = rng.integers(0, 2, size=(nz, ny, nx + 1)).astype(np.uint8) * 2
+    bz[rng.random(bz.shape) < 0.1] = 3
+    by[rng.random(by.shape) < 0.1] = 3
+    bx[rng.random(bx.shape) < 0.1] = 3
+    for a, sl in ((bz, (0,)), (bz, (-1,)), (by, (slice(None), 0)), (by, (slice(None), -1)),
+                  (bx, (Ellipsis, 0)), (bx, (Ellipsis, -1))):
+        a[sl] = 1
+    v = np.zeros(shape, np.uint8)
+    v[0] = 1
+    v[rng.random(shape) < 0.1] = 2
+    d = (3e-4 + 2e-4 * rng.random(), 3e-4 + 2e-4 * rng.random(), 3e-4 + 2e-4 * rng.random())
+    materials = [(0, 50.0, 50.0, 50.0, 7850.0, 470.0, 0), (0, 0.4, 0.4, 0.4, 1200.0, 1300.0, 0),
+                 (0, 5.0, 5.0, 5.0, 1550.0, 1200.0, 0), (1, 0, 0, 0, 0, 0, 0)]
+    boundaries = [(cases.BND_CONDUCTING, 0.0), (cases.BND_INSULATING, 0.0),
+                  (cases.BND_CONDUCTING, 0.0), (cases.BND_THIN, 800.0)]
+    volumetric = [(0,), (1, 0.0, 1e4 / d[0]), (2, 0.0, 3e-3, 2e7)]
+    T0 = 293.0 + rng.random(shape)
+    T0[m == 3] = 290.0
+    return dict(shape=shape, d=d, dt=1e-3, t0=0.0, nsteps=4, materials=materials,
+                boundaries=boundaries, volumetric=volumetric, maps=(m, bz, by, bx, v), T0=T0)
+
+
+SHAPES = [(3, 5, 7), (2, 1, 1), (1, 4, 9), (5, 3, 64), (4, 6, 65), (3, 70, 130), (70, 3, 5),
+          (6, 4, 256), (9, 17, 200), (130, 2, 33), (4, 300, 3), (260, 3, 2), (2, 2, 513),
+          (3, 512, 2), (517, 2, 3), (2, 1000, 2), (1024, 1, 1), (2, 3, 1024)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_random_problems_match_oracle(hs, shape):
+    """Random materials (incl. FIXED), conducting / thin / insulating faces and
+    sources on grids whose line lengths exercise every kernel variant
+    (1..1024 cells on every axis, non-multiples of 64)."""
+    case = _random_case(shape, seed=sum(shape))
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    ref = oracle.run_ref(case)
+    T = torch.from_numpy(case["T0"]).cuda()
+    rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
+    for n in range(case["nsteps"]):
+        T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"], args["volumetric"])
+        assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, n
+
+
+def test_run_many_equals_stepwise(hs):
+    case = load_golden("tiny_mixed")
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    T = torch.from_numpy(case["T0"]).cuda()
+    hist = []
+    for n in range(case["nsteps"]):
+        T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"], args["volumetric"])
+        hist.append(hs.surface_temperature.insulating_z_min_surface_temperature(T, case["d"][0]))
+    T2, h2 = hs.run_adi_steps_many(P, 0.0, case["dt"], case["nsteps"],
+                                   torch.from_numpy(case["T0"]).cuda(), args["vmap"],
+                                   args["volumetric"], surface_history=True)
+    assert torch.equal(T, T2)
+    assert torch.equal(torch.stack(hist), h2)
+
+
+def test_zdecay_source_matches_upstream(hs):
+    case = load_golden("tiny_aniso")  # carries a z-decaying impulse at t=0
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    ref = case["fixture"]["traj"]
+    T, _ = hs.run_adi_steps_many(P, 0.0, case["dt"], case["nsteps"],
+                                 torch.from_numpy(case["T0"]).cuda(), args["vmap"],
+                                 args["volumetric"])
+    rise = np.abs(ref - case["T0"]).max()
+    assert _rel(T.cpu().numpy(), ref[-1], rise) < TOL
+
+
+def test_deterministic(hs):
+    """No atomics in the step: two runs are bitwise equal."""
+    case = load_golden("laminate_mini")
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    outs = []
+    for _ in range(2):
+        T, _h = hs.run_adi_steps_many(P, 0.0, case["dt"], 30, torch.from_numpy(case["T0"]).cuda(),
+                                      args["vmap"], args["volumetric"])
+        outs.append(T)
+    assert torch.equal(outs[0], outs[1])
