@@ -73,7 +73,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, nargs=3, default=(256, 256, 256))
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 256^3 float64 field 20 times (scales HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.rand(256 ** 3, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(20):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": a.numel() * 8, "copies": 20}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -94,6 +107,11 @@ def main():
     n = nz * ny * nx
     dt = case["dt"]
     T = torch.from_numpy(case["T0"]).to(dev)
+    if not args.no_tune:
+        t0 = time.time()
+        cfg = hs.autotune.tune(P, T, use_cache=False, verbose=args.verbose)
+        log("autotune %.2f s: %s" % (time.time() - t0, cfg))
+    tuned = _lib.plan_info(P.plan)
     # warmup: the flash step plus W-1 more
     T, _ = hs.run_adi_steps_many(P, 0.0, dt, max(args.warmup, 1), T, pa["vmap"], pa["volumetric"])
     K = args.steps
@@ -135,7 +153,9 @@ def main():
            "config": {"workload": "two-layer steel/foam stack %dx%dx%d (z,y,x), dx=0.5 mm, thin "
                                   "layer h=250 W/m^2K, flash %.1f J/m^2, dt=0.5 ms, FP64, surface "
                                   "readout every step" % (nz, ny, nx, case["flash_J_m2"]),
-                      "cells": n, "parallelism": "single GPU"},
+                      "cells": n, "parallelism": "single GPU",
+                      "launch": {k: tuned[k] for k in ("x_cpl", "x_lanes", "y_seg", "y_lines",
+                                                       "z_seg", "z_lines", "mode")}},
            "roofline": roofline}
     del T, hist
     if not args.no_cpu:

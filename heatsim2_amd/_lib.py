@@ -30,9 +30,9 @@ class HS2Error(RuntimeError):
 
 
 # every symbol include/heatsim2_amd.h declares
-EXPORTS = ("hs2_plan_create", "hs2_plan_destroy", "hs2_plan_set_compact", "hs2_plan_expand",
-           "hs2_step", "hs2_run", "hs2_run_timed", "hs2_surface_temperature", "hs2_last_error",
-           "hs2_version")
+EXPORTS = ("hs2_plan_create", "hs2_plan_destroy", "hs2_plan_set_compact", "hs2_plan_set_dense",
+           "hs2_plan_expand", "hs2_plan_set_config", "hs2_plan_info", "hs2_step", "hs2_run",
+           "hs2_run_timed", "hs2_surface_temperature", "hs2_last_error", "hs2_version")
 
 _lib = None
 
@@ -53,8 +53,14 @@ def load():
     L.hs2_plan_destroy.restype = None
     L.hs2_plan_set_compact.argtypes = [vp, vp, vp, vp, vp, i32, i32, vp, vp, vp]
     L.hs2_plan_set_compact.restype = i32
+    L.hs2_plan_set_dense.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.hs2_plan_set_dense.restype = i32
     L.hs2_plan_expand.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.hs2_plan_expand.restype = i32
+    L.hs2_plan_set_config.argtypes = [vp, i32, i32, i32, i32, i32, i32]
+    L.hs2_plan_set_config.restype = i32
+    L.hs2_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.hs2_plan_info.restype = i32
     L.hs2_step.argtypes = [vp, vp, vp, vp, vp, i32, vp, vp, vp]
     L.hs2_step.restype = i32
     L.hs2_run.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp]
@@ -108,3 +114,11 @@ def surface_temperature_device(T, dz):
     check("hs2_surface_temperature",
           load().hs2_surface_temperature(ptr(T), nz, ny, nx, float(dz), ptr(out), stream_handle()))
     return out
+
+
+def plan_info(plan):
+    info = (ctypes.c_int * 10)()
+    check("hs2_plan_info", load().hs2_plan_info(plan, info))
+    keys = ("mode", "classes_z", "classes_y", "classes_x", "x_cpl", "x_lanes", "y_seg",
+            "y_lines", "z_seg", "z_lines")
+    return dict(zip(keys, list(info)))

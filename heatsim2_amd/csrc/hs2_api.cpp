@@ -5,7 +5,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/heatsim2_amd.h"
@@ -31,9 +33,10 @@ int fail(int code, const char *fmt, ...) {
     if (e_ != hipSuccess) return fail(HS2_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-// LDS budget per block for the tables (doubles); beyond this the compact
-// description is refused.
-constexpr int kMaxTabDoubles = 4096;
+enum { MODE_NONE = 0, MODE_CLASS = 1, MODE_DENSE = 2 };
+
+constexpr int kCheckBlocks = 256;
+constexpr int kMaxBlockThreads = 512;  // __launch_bounds__ of the strided kernel
 
 }  // namespace
 
@@ -41,22 +44,27 @@ struct hs2_plan {
   hs2_grid_desc grid{};
   int n = 0;
   int device = 0;
-  hs2::Compact cv{};
-  double *d_tab = nullptr;
-  bool ready = false;
-  double *d_ws = nullptr;     // n doubles: T* then T**
-  double *d_vtab = nullptr;   // kMaxVtab doubles (hs2_step)
-  double *d_vsteps = nullptr; // per-step source tables (hs2_run)
-  size_t vsteps_cap = 0;      // doubles
-  int *d_check = nullptr;     // per-block maxima of the maps (validation)
-  int cpl = 0;
-  int seg[2] = {0, 0};        // strided config per axis index 0=z, 1=y
-  int cw[2] = {0, 0};
+  int mode = MODE_NONE;
+  hs2::ClassView cv{};
+  hs2::DenseView dv{};
+  int ncls[3] = {0, 0, 0};
+  uint8_t *d_cls = nullptr;     // 3n class ids (class mode)
+  double *d_tab = nullptr;      // class tables
+  uint8_t *d_lut = nullptr;     // classification lookup tables
+  double *d_dense = nullptr;    // plan-built dense arrays: M, gz, gy, gx (4n)
+  uint8_t *d_densew = nullptr;  // plan-built dense row weights (n)
+  double *d_ws = nullptr;       // n doubles: T* then T**
+  double *d_vtab = nullptr;     // kMaxVtab doubles (hs2_step)
+  double *d_vsteps = nullptr;   // per-step source tables (hs2_run)
+  size_t vsteps_cap = 0;        // doubles
+  int *d_check = nullptr;       // per-block maxima of the maps (validation)
+  hs2::XConfig xcfg{0, 0};
+  hs2::SConfig scfg[2] = {{0, 0}, {0, 0}};  // [0] z, [1] y
 };
 
 namespace {
 
-int strided_config(int L, int *seg, int *cw) {
+int default_strided(int L, hs2::SConfig *cfg) {
   int s;
   if (L <= 4) s = 2;
   else if (L <= 16) s = 4;
@@ -66,11 +74,27 @@ int strided_config(int L, int *seg, int *cw) {
   else return 0;
   const int P = (L + s - 1) / s;
   int w = 64;
-  while (w > 8 && P * w > 512) w >>= 1;
-  if (P * w > 512) return 0;  // __launch_bounds__(512) of the strided kernel
-  *seg = s;
-  *cw = w;
+  while (w > 8 && P * w > kMaxBlockThreads) w >>= 1;
+  if (P * w > kMaxBlockThreads) return 0;
+  *cfg = {s, w};
   return 1;
+}
+
+bool strided_ok(int L, hs2::SConfig c) {
+  const bool seg_ok = c.seg == 2 || c.seg == 4 || c.seg == 8 || c.seg == 16 || c.seg == 32;
+  const bool cw_ok = c.cw == 8 || c.cw == 16 || c.cw == 32 || c.cw == 64;
+  if (!seg_ok || !cw_ok) return false;
+  const int P = (L + c.seg - 1) / c.seg;
+  return P * c.cw <= kMaxBlockThreads;
+}
+
+bool x_ok(int nx, hs2::XConfig c) {
+  static const int combos[][2] = {{1, 16}, {2, 16}, {4, 16}, {8, 16}, {16, 16},
+                                  {2, 32}, {4, 32}, {8, 32}, {16, 32},
+                                  {2, 64}, {4, 64}, {8, 64}, {16, 64}};
+  for (auto &cb : combos)
+    if (cb[0] == c.cpl && cb[1] == c.g) return c.cpl * c.g >= nx;
+  return false;
 }
 
 // per-block maximum of a uint8 map (out[blockIdx.x]); blockDim.x == 256
@@ -89,9 +113,32 @@ __global__ void k_map_max(const uint8_t *a, size_t n, int *out) {
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-constexpr int kCheckBlocks = 256;
+// dense arrays from the maps and the full conductance table (used when an
+// axis has more than 256 distinct coefficient classes)
+__global__ void k_dense_from_maps(hs2::Grid g, const uint8_t *__restrict__ mat,
+                                  const uint8_t *__restrict__ bz, const uint8_t *__restrict__ by,
+                                  const uint8_t *__restrict__ bx, const double *__restrict__ gtab,
+                                  const double *__restrict__ mtab, const uint8_t *__restrict__ comp,
+                                  int nmat, double *M, uint8_t *w, double *gz, double *gy,
+                                  double *gx) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int n = nx * ny * nz, sxy = nx * ny;
+  const size_t ax = (size_t)nmat * nmat;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const int k = c / sxy, rem = c - k * sxy, j = rem / nx, i = rem - j * nx;
+    const int m = mat[c];
+    M[c] = mtab[m];
+    w[c] = comp[m];
+    gz[c] = k > 0 ? gtab[(bz[c] * 3 + 0) * ax + (size_t)mat[c - sxy] * nmat + m] : 0.0;
+    gy[c] = j > 0 ? gtab[(by[(k * (ny + 1) + j) * nx + i] * 3 + 1) * ax +
+                         (size_t)mat[c - nx] * nmat + m]
+                  : 0.0;
+    gx[c] = i > 0 ? gtab[(bx[(k * ny + j) * (nx + 1) + i] * 3 + 2) * ax +
+                         (size_t)mat[c - 1] * nmat + m]
+                  : 0.0;
+  }
+}
 
-// largest entry of a device uint8 map
 int map_max(const uint8_t *d_map, size_t n, int *d_scratch, int *result) {
   hipLaunchKernelGGL(k_map_max, dim3(kCheckBlocks), dim3(256), 0, 0, d_map, n, d_scratch);
   HIP_TRY(hipGetLastError());
@@ -101,15 +148,43 @@ int map_max(const uint8_t *d_map, size_t n, int *d_scratch, int *result) {
   return HS2_OK;
 }
 
+hs2::Grid grid_of(const hs2_plan *p) { return hs2::Grid{p->grid.nz, p->grid.ny, p->grid.nx}; }
+
+hipError_t sweep_x(const hs2_plan *p, const double *Tn, double *Ts, const uint8_t *vmap,
+                   const double *vtab, const double *sfield, hipStream_t s) {
+  return p->mode == MODE_CLASS
+             ? hs2::launch_sweep_x(p->xcfg, grid_of(p), p->cv, Tn, Ts, vmap, vtab, sfield, s)
+             : hs2::launch_sweep_x(p->xcfg, grid_of(p), p->dv, Tn, Ts, vmap, vtab, sfield, s);
+}
+
+hipError_t sweep_s(const hs2_plan *p, int axis, const double *Tn, const double *Y, double *X,
+                   double *surf, hipStream_t s) {
+  const hs2::SConfig c = p->scfg[axis];
+  return p->mode == MODE_CLASS
+             ? hs2::launch_sweep_strided(c, grid_of(p), p->cv, axis, Tn, Y, X, surf, p->grid.dz, s)
+             : hs2::launch_sweep_strided(c, grid_of(p), p->dv, axis, Tn, Y, X, surf, p->grid.dz, s);
+}
+
 int run_step(hs2_plan *p, const double *Tin, double *Tout, const uint8_t *vmap,
              const double *d_vtab, const double *sfield, double *surf, hipStream_t s) {
-  const hs2::Grid g{p->grid.nz, p->grid.ny, p->grid.nx};
-  HIP_TRY(hs2::launch_sweep_x(p->cpl, g, p->cv, Tin, p->d_ws, vmap, d_vtab, sfield, s));
-  HIP_TRY(hs2::launch_sweep_strided(p->seg[1], p->cw[1], g, p->cv, 1, Tin, p->d_ws, p->d_ws,
-                                    nullptr, p->grid.dz, s));
-  HIP_TRY(hs2::launch_sweep_strided(p->seg[0], p->cw[0], g, p->cv, 0, Tin, p->d_ws, Tout, surf,
-                                    p->grid.dz, s));
+  HIP_TRY(sweep_x(p, Tin, p->d_ws, vmap, d_vtab, sfield, s));
+  HIP_TRY(sweep_s(p, 1, Tin, p->d_ws, p->d_ws, nullptr, s));
+  HIP_TRY(sweep_s(p, 0, Tin, p->d_ws, Tout, surf, s));
   return HS2_OK;
+}
+
+void free_coefficients(hs2_plan *p) {
+  (void)hipFree(p->d_cls);
+  (void)hipFree(p->d_tab);
+  (void)hipFree(p->d_lut);
+  (void)hipFree(p->d_dense);
+  (void)hipFree(p->d_densew);
+  p->d_cls = nullptr;
+  p->d_tab = nullptr;
+  p->d_lut = nullptr;
+  p->d_dense = nullptr;
+  p->d_densew = nullptr;
+  p->mode = MODE_NONE;
 }
 
 }  // namespace
@@ -118,7 +193,7 @@ extern "C" {
 
 const char *hs2_last_error(void) { return g_last_error.c_str(); }
 
-const char *hs2_version(void) { return "heatsim2_amd 0.1 (gfx950, fp64)"; }
+const char *hs2_version(void) { return "heatsim2_amd 0.2 (gfx950, fp64)"; }
 
 int hs2_plan_create(const hs2_grid_desc *grid, hs2_plan **out) {
   if (!grid || !out) return fail(HS2_E_ARG, "hs2_plan_create: null argument");
@@ -132,13 +207,13 @@ int hs2_plan_create(const hs2_grid_desc *grid, hs2_plan **out) {
   if ((long long)(G.nz + 1) * G.ny * G.nx >= (1LL << 31) ||
       (long long)G.nz * G.ny * (G.nx + 1) >= (1LL << 31))
     return fail(HS2_E_UNSUPPORTED, "hs2_plan_create: grid of %lld cells exceeds 2^31 indices", n);
-  const int cpl = hs2::x_cpl_for(G.nx);
-  if (!cpl) return fail(HS2_E_UNSUPPORTED, "hs2_plan_create: nx=%d > 1024 unsupported", G.nx);
+  const hs2::XConfig xc = hs2::x_config_for(G.nx);
+  if (!xc.cpl) return fail(HS2_E_UNSUPPORTED, "hs2_plan_create: nx=%d > 1024 unsupported", G.nx);
   hs2_plan *p = new hs2_plan();
   p->grid = G;
   p->n = (int)n;
-  p->cpl = cpl;
-  if (!strided_config(G.ny, &p->seg[1], &p->cw[1]) || !strided_config(G.nz, &p->seg[0], &p->cw[0])) {
+  p->xcfg = xc;
+  if (!default_strided(G.nz, &p->scfg[0]) || !default_strided(G.ny, &p->scfg[1])) {
     delete p;
     return fail(HS2_E_UNSUPPORTED, "hs2_plan_create: ny=%d / nz=%d > 1024 unsupported", G.ny, G.nz);
   }
@@ -156,10 +231,10 @@ int hs2_plan_create(const hs2_grid_desc *grid, hs2_plan **out) {
 
 void hs2_plan_destroy(hs2_plan *p) {
   if (!p) return;
+  free_coefficients(p);
   (void)hipFree(p->d_ws);
   (void)hipFree(p->d_vtab);
   (void)hipFree(p->d_vsteps);
-  (void)hipFree(p->d_tab);
   (void)hipFree(p->d_check);
   delete p;
 }
@@ -171,23 +246,18 @@ int hs2_plan_set_compact(hs2_plan *p, const uint8_t *d_mat, const uint8_t *d_bz,
     return fail(HS2_E_ARG, "hs2_plan_set_compact: null argument");
   if (nmat < 1 || nmat > 256 || nb < 1 || nb > 256)
     return fail(HS2_E_ARG, "hs2_plan_set_compact: nmat=%d nb=%d must be in 1..256", nmat, nb);
-  const long long glen = 3LL * nb * nmat * nmat;
-  const long long tab_len = glen + 2LL * nmat;
-  if (tab_len > kMaxTabDoubles)
-    return fail(HS2_E_UNSUPPORTED,
-                "hs2_plan_set_compact: table of %lld doubles (nb=%d, nmat=%d) exceeds %d", tab_len,
-                nb, nmat, kMaxTabDoubles);
+  const size_t lutlen = (size_t)nb * nmat * nmat;
+  const size_t glen = 3 * lutlen;
   for (int m = 0; m < nmat; m++) {
     if (h_compute[m] > 1) return fail(HS2_E_ARG, "hs2_plan_set_compact: compute flag not 0/1");
-    if (!(h_mtab[m] > 0.0))
+    if (!(h_mtab[m] > 0.0) || h_mtab[m] > 1e300)
       return fail(HS2_E_ARG, "hs2_plan_set_compact: M[%d]=%g must be positive", m, h_mtab[m]);
   }
-  for (long long t = 0; t < glen; t++)
+  for (size_t t = 0; t < glen; t++)
     if (!(h_gtab[t] >= 0.0) || h_gtab[t] > 1e300)
-      return fail(HS2_E_ARG, "hs2_plan_set_compact: conductance table entry %lld = %g invalid", t,
+      return fail(HS2_E_ARG, "hs2_plan_set_compact: conductance table entry %zu = %g invalid", t,
                   h_gtab[t]);
   const hs2_grid_desc &G = p->grid;
-  // every map entry must index the tables
   const size_t nzf = (size_t)(G.nz + 1) * G.ny * G.nx, nyf = (size_t)G.nz * (G.ny + 1) * G.nx,
                nxf = (size_t)G.nz * G.ny * (G.nx + 1);
   int mx[4], rc;
@@ -202,38 +272,162 @@ int hs2_plan_set_compact(hs2_plan *p, const uint8_t *d_mat, const uint8_t *d_bz,
     if (mx[t] >= nb)
       return fail(HS2_E_ARG, "hs2_plan_set_compact: %s face map holds %d >= nb=%d",
                   t == 1 ? "z" : (t == 2 ? "y" : "x"), mx[t], nb);
-  std::vector<double> tab((size_t)tab_len);
-  std::memcpy(tab.data(), h_gtab, sizeof(double) * (size_t)glen);
-  for (int m = 0; m < nmat; m++) {
-    tab[(size_t)glen + m] = h_mtab[m];
-    tab[(size_t)glen + nmat + m] = h_compute[m] ? 1.0 : 0.0;
+
+  // Coefficient classes per axis: a cell's entry is (M, w, g of its minus
+  // face); g is 0 on the domain edge.  Distinct entries are numbered.
+  std::vector<uint8_t> lut(3 * lutlen + 3 * (size_t)nmat, 0);
+  std::vector<double> tabs;
+  int off[4] = {0, 0, 0, 0};
+  bool fits = true;
+  for (int a = 0; a < 3 && fits; a++) {
+    std::map<std::tuple<double, double, double>, int> ids;
+    std::vector<std::tuple<double, double, double>> order;
+    auto id_of = [&](double M, double w, double gv) -> int {
+      auto key = std::make_tuple(M, w, gv);
+      auto it = ids.find(key);
+      if (it != ids.end()) return it->second;
+      const int id = (int)order.size();
+      ids.emplace(key, id);
+      order.push_back(key);
+      return id;
+    };
+    for (int m = 0; m < nmat; m++) {
+      const int id = id_of(h_mtab[m], h_compute[m] ? 1.0 : 0.0, 0.0);
+      if (id >= hs2::kMaxClass) { fits = false; break; }
+      lut[3 * lutlen + (size_t)a * nmat + m] = (uint8_t)id;
+    }
+    for (int b = 0; b < nb && fits; b++)
+      for (int ma = 0; ma < nmat && fits; ma++)
+        for (int m = 0; m < nmat; m++) {
+          const double gv = h_gtab[(((size_t)b * 3 + a) * nmat + ma) * nmat + m];
+          const int id = id_of(h_mtab[m], h_compute[m] ? 1.0 : 0.0, gv);
+          if (id >= hs2::kMaxClass) { fits = false; break; }
+          lut[(size_t)a * lutlen + ((size_t)b * nmat + ma) * nmat + m] = (uint8_t)id;
+        }
+    off[a] = (int)tabs.size();
+    for (auto &e : order) {
+      tabs.push_back(std::get<0>(e));
+      tabs.push_back(std::get<1>(e));
+      tabs.push_back(std::get<2>(e));
+      tabs.push_back(0.0);
+    }
+    p->ncls[a] = (int)order.size();
   }
-  // tables must fit the LDS of the strided kernels next to their reduced systems
-  const hs2::Grid g{G.nz, G.ny, G.nx};
-  hs2::Compact cv{d_mat, d_bz, d_by, d_bx, nullptr, nmat, nb, (int)tab_len};
-  for (int ax = 0; ax < 2; ax++) {
-    const int L = ax == 1 ? G.ny : G.nz;
-    const int P = (L + p->seg[ax] - 1) / p->seg[ax];
-    if (hs2::strided_lds_bytes(cv, P, p->cw[ax]) > 64 * 1024)
-      return fail(HS2_E_UNSUPPORTED, "hs2_plan_set_compact: LDS budget exceeded");
+  off[3] = (int)tabs.size();
+  free_coefficients(p);
+  const hs2::Grid g = grid_of(p);
+  const size_t n = (size_t)p->n;
+  if (fits) {
+    HIP_TRY(hipMalloc(&p->d_cls, 3 * n));
+    HIP_TRY(hipMalloc(&p->d_tab, sizeof(double) * tabs.size()));
+    HIP_TRY(hipMalloc(&p->d_lut, lut.size()));
+    HIP_TRY(hipMemcpy(p->d_tab, tabs.data(), sizeof(double) * tabs.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_lut, lut.data(), lut.size(), hipMemcpyHostToDevice));
+    uint8_t *cz = p->d_cls, *cy = p->d_cls + n, *cx = p->d_cls + 2 * n;
+    HIP_TRY(hs2::launch_classify(g, d_mat, d_bz, d_by, d_bx, p->d_lut, p->d_lut + 3 * lutlen, nmat,
+                                 (int)lutlen, cz, cy, cx, 0));
+    HIP_TRY(hipDeviceSynchronize());
+    p->cv.cls[0] = cz;
+    p->cv.cls[1] = cy;
+    p->cv.cls[2] = cx;
+    p->cv.tab = p->d_tab;
+    for (int t = 0; t < 4; t++) p->cv.off[t] = off[t];
+    p->mode = MODE_CLASS;
+    return HS2_OK;
   }
-  (void)g;
-  if (p->d_tab) (void)hipFree(p->d_tab);
-  p->d_tab = nullptr;
-  HIP_TRY(hipMalloc(&p->d_tab, sizeof(double) * tab.size()));
-  HIP_TRY(hipMemcpy(p->d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
-  cv.tab = p->d_tab;
-  p->cv = cv;
-  p->ready = true;
+  // more than 256 classes on an axis: per-cell float64 coefficients
+  double *d_gtab = nullptr, *d_mtab = nullptr;
+  uint8_t *d_comp = nullptr;
+  hipError_t e = hipMalloc(&p->d_dense, sizeof(double) * 4 * n);
+  if (e == hipSuccess) e = hipMalloc(&p->d_densew, n);
+  if (e == hipSuccess) e = hipMalloc(&d_gtab, sizeof(double) * glen);
+  if (e == hipSuccess) e = hipMalloc(&d_mtab, sizeof(double) * nmat);
+  if (e == hipSuccess) e = hipMalloc(&d_comp, nmat);
+  if (e == hipSuccess) e = hipMemcpy(d_gtab, h_gtab, sizeof(double) * glen, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_mtab, h_mtab, sizeof(double) * nmat, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_comp, h_compute, nmat, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_dense_from_maps, dim3(blocks), dim3(256), 0, 0, g, d_mat, d_bz, d_by, d_bx,
+                       d_gtab, d_mtab, d_comp, nmat, p->d_dense, p->d_densew, p->d_dense + n,
+                       p->d_dense + 2 * n, p->d_dense + 3 * n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  (void)hipFree(d_gtab);
+  (void)hipFree(d_mtab);
+  (void)hipFree(d_comp);
+  if (e != hipSuccess) {
+    free_coefficients(p);
+    return fail(HS2_E_HIP, "hs2_plan_set_compact (dense fallback): %s", hipGetErrorString(e));
+  }
+  p->dv.M = p->d_dense;
+  p->dv.w = p->d_densew;
+  p->dv.g[0] = p->d_dense + n;
+  p->dv.g[1] = p->d_dense + 2 * n;
+  p->dv.g[2] = p->d_dense + 3 * n;
+  p->mode = MODE_DENSE;
   return HS2_OK;
 }
 
-int hs2_plan_expand(hs2_plan *p, double *d_gz, double *d_gy, double *d_gx, double *d_M,
-                    uint8_t *d_cflag, void *stream) {
+int hs2_plan_set_dense(hs2_plan *p, const double *d_M, const uint8_t *d_w, const double *d_gz,
+                       const double *d_gy, const double *d_gx) {
+  if (!p || !d_M || !d_w || !d_gz || !d_gy || !d_gx)
+    return fail(HS2_E_ARG, "hs2_plan_set_dense: null argument");
+  free_coefficients(p);
+  p->dv.M = d_M;
+  p->dv.w = d_w;
+  p->dv.g[0] = d_gz;
+  p->dv.g[1] = d_gy;
+  p->dv.g[2] = d_gx;
+  p->mode = MODE_DENSE;
+  return HS2_OK;
+}
+
+int hs2_plan_info(const hs2_plan *p, int *info) {
+  if (!p || !info) return fail(HS2_E_ARG, "hs2_plan_info: null argument");
+  info[0] = p->mode;
+  info[1] = p->ncls[0];
+  info[2] = p->ncls[1];
+  info[3] = p->ncls[2];
+  info[4] = p->xcfg.cpl;
+  info[5] = p->xcfg.g;
+  info[6] = p->scfg[1].seg;
+  info[7] = p->scfg[1].cw;
+  info[8] = p->scfg[0].seg;
+  info[9] = p->scfg[0].cw;
+  return HS2_OK;
+}
+
+int hs2_plan_set_config(hs2_plan *p, int x_cpl, int x_lanes, int y_seg, int y_lines, int z_seg,
+                        int z_lines) {
+  if (!p) return fail(HS2_E_ARG, "hs2_plan_set_config: null plan");
+  const hs2::XConfig xc{x_cpl, x_lanes};
+  const hs2::SConfig yc{y_seg, y_lines}, zc{z_seg, z_lines};
+  if (!x_ok(p->grid.nx, xc))
+    return fail(HS2_E_ARG, "hs2_plan_set_config: x config %dx%d invalid for nx=%d", x_cpl, x_lanes,
+                p->grid.nx);
+  if (!strided_ok(p->grid.ny, yc))
+    return fail(HS2_E_ARG, "hs2_plan_set_config: y config seg=%d lines=%d invalid for ny=%d",
+                y_seg, y_lines, p->grid.ny);
+  if (!strided_ok(p->grid.nz, zc))
+    return fail(HS2_E_ARG, "hs2_plan_set_config: z config seg=%d lines=%d invalid for nz=%d",
+                z_seg, z_lines, p->grid.nz);
+  p->xcfg = xc;
+  p->scfg[1] = yc;
+  p->scfg[0] = zc;
+  return HS2_OK;
+}
+
+int hs2_plan_expand(hs2_plan *p, double *d_M, double *d_w, double *d_gz, double *d_gy,
+                    double *d_gx, void *stream) {
   if (!p) return fail(HS2_E_ARG, "hs2_plan_expand: null plan");
-  if (!p->ready) return fail(HS2_E_STATE, "hs2_plan_expand: coefficients not set");
-  const hs2::Grid g{p->grid.nz, p->grid.ny, p->grid.nx};
-  HIP_TRY(hs2::launch_expand(g, p->cv, d_gz, d_gy, d_gx, d_M, d_cflag, (hipStream_t)stream));
+  if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_plan_expand: coefficients not set");
+  hipStream_t s = (hipStream_t)stream;
+  if (p->mode == MODE_CLASS)
+    HIP_TRY(hs2::launch_expand(grid_of(p), p->cv, d_M, d_w, d_gz, d_gy, d_gx, s));
+  else
+    HIP_TRY(hs2::launch_expand(grid_of(p), p->dv, d_M, d_w, d_gz, d_gy, d_gx, s));
   return HS2_OK;
 }
 
@@ -241,8 +435,8 @@ int hs2_step(hs2_plan *p, const double *d_Tin, double *d_Tout, const uint8_t *d_
              const double *h_vtab, int nvtab, const double *d_sfield, double *d_surf,
              void *stream) {
   if (!p || !d_Tin || !d_Tout) return fail(HS2_E_ARG, "hs2_step: null argument");
+  if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_step: coefficients not set");
   if (d_surf && p->grid.nz < 2) return fail(HS2_E_ARG, "hs2_step: surface readout needs nz >= 2");
-  if (!p->ready) return fail(HS2_E_STATE, "hs2_step: coefficients not set");
   if ((d_vmap != nullptr) != (h_vtab != nullptr))
     return fail(HS2_E_ARG, "hs2_step: class map and class table go together");
   if (d_vmap && (nvtab < 1 || nvtab > hs2::kMaxVtab))
@@ -262,13 +456,12 @@ int hs2_step(hs2_plan *p, const double *d_Tin, double *d_Tout, const uint8_t *d_
 int hs2_run(hs2_plan *p, double *d_T, int nsteps, const uint8_t *d_vmap,
             const double *h_vtab_steps, int nvtab, double *d_surf_hist, void *stream) {
   if (!p || !d_T) return fail(HS2_E_ARG, "hs2_run: null argument");
-  if (!p->ready) return fail(HS2_E_STATE, "hs2_run: coefficients not set");
+  if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_run: coefficients not set");
   if (nsteps < 0) return fail(HS2_E_ARG, "hs2_run: nsteps=%d", nsteps);
   if (d_surf_hist && p->grid.nz < 2) return fail(HS2_E_ARG, "hs2_run: surface readout needs nz >= 2");
   if (h_vtab_steps && (!d_vmap || nvtab < 1 || nvtab > hs2::kMaxVtab))
     return fail(HS2_E_ARG, "hs2_run: per-step source tables need the class map and 1..256 classes");
   hipStream_t s = (hipStream_t)stream;
-  // gather the steps that carry a source
   std::vector<int> active_row(nsteps, -1);
   std::vector<double> rows;
   if (h_vtab_steps) {
@@ -283,18 +476,16 @@ int hs2_run(hs2_plan *p, double *d_T, int nsteps, const uint8_t *d_vmap,
     }
   }
   if (!rows.empty()) {
+    HIP_TRY(hipStreamSynchronize(s));
     if (rows.size() > p->vsteps_cap) {
-      HIP_TRY(hipStreamSynchronize(s));
       (void)hipFree(p->d_vsteps);
       p->d_vsteps = nullptr;
       p->vsteps_cap = 0;
       HIP_TRY(hipMalloc(&p->d_vsteps, sizeof(double) * rows.size()));
       p->vsteps_cap = rows.size();
     }
-    HIP_TRY(hipMemcpyAsync(p->d_vsteps, rows.data(), sizeof(double) * rows.size(),
-                           hipMemcpyHostToDevice, s));
-    // the host vector must outlive the copy
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(p->d_vsteps, rows.data(), sizeof(double) * rows.size(),
+                      hipMemcpyHostToDevice));
   }
   const size_t sxy = (size_t)p->grid.ny * p->grid.nx;
   for (int t = 0; t < nsteps; t++) {
@@ -309,24 +500,24 @@ int hs2_run(hs2_plan *p, double *d_T, int nsteps, const uint8_t *d_vmap,
 
 int hs2_run_timed(hs2_plan *p, double *d_T, int nsteps, double *kernel_ms, void *stream) {
   if (!p || !d_T || !kernel_ms || nsteps < 1) return fail(HS2_E_ARG, "hs2_run_timed: bad argument");
-  if (!p->ready) return fail(HS2_E_STATE, "hs2_run_timed: coefficients not set");
+  if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_run_timed: coefficients not set");
   hipStream_t s = (hipStream_t)stream;
-  const hs2::Grid g{p->grid.nz, p->grid.ny, p->grid.nx};
   std::vector<hipEvent_t> ev((size_t)nsteps * 4, nullptr);
   int rc = HS2_OK;
   for (auto &e : ev) {
-    if (hipEventCreate(&e) != hipSuccess) { rc = fail(HS2_E_HIP, "hs2_run_timed: hipEventCreate"); break; }
+    if (hipEventCreate(&e) != hipSuccess) {
+      rc = fail(HS2_E_HIP, "hs2_run_timed: hipEventCreate");
+      break;
+    }
   }
   for (int t = 0; t < nsteps && rc == HS2_OK; t++) {
     hipEvent_t *e = &ev[(size_t)t * 4];
     hipError_t err = hipEventRecord(e[0], s);
-    if (err == hipSuccess) err = hs2::launch_sweep_x(p->cpl, g, p->cv, d_T, p->d_ws, nullptr, nullptr, nullptr, s);
+    if (err == hipSuccess) err = sweep_x(p, d_T, p->d_ws, nullptr, nullptr, nullptr, s);
     if (err == hipSuccess) err = hipEventRecord(e[1], s);
-    if (err == hipSuccess)
-      err = hs2::launch_sweep_strided(p->seg[1], p->cw[1], g, p->cv, 1, d_T, p->d_ws, p->d_ws, nullptr, p->grid.dz, s);
+    if (err == hipSuccess) err = sweep_s(p, 1, d_T, p->d_ws, p->d_ws, nullptr, s);
     if (err == hipSuccess) err = hipEventRecord(e[2], s);
-    if (err == hipSuccess)
-      err = hs2::launch_sweep_strided(p->seg[0], p->cw[0], g, p->cv, 0, d_T, p->d_ws, d_T, nullptr, p->grid.dz, s);
+    if (err == hipSuccess) err = sweep_s(p, 0, d_T, p->d_ws, d_T, nullptr, s);
     if (err == hipSuccess) err = hipEventRecord(e[3], s);
     if (err != hipSuccess) rc = fail(HS2_E_HIP, "hs2_run_timed: %s", hipGetErrorString(err));
   }

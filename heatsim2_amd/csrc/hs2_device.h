@@ -1,12 +1,20 @@
 // Device-side building blocks shared by the heatsim2_amd kernels (gfx950).
 //
 // Coefficient model.  For a COMPUTE cell c the explicit operator on axis a is
-//   (L_a w)_c = g_a-(c) (w_{c-e_a} - w_c) + g_a+(c) (w_{c+e_a} - w_c)
-// and M_c = rho*c/dt; a FIXED cell has M = 1 and no couplings.  g is a
-// property of the face (the same number from both COMPUTE sides), looked up
-// in a per-plan table indexed by (face class, axis, minus material, plus
-// material) -- the numbers the upstream setup derives symbolically per cell
-// (crank_nicolson.pyx:298-386).  Outer faces never carry flux.
+//   (L_a u)_c = g_a(c) (u_{c-e_a} - u_c) + g_a(c+e_a) (u_{c+e_a} - u_c)
+// where g_a(c) is the conductance per volume (W m^-3 K^-1) of the face
+// between c-e_a and c (0 on the outer faces), and M_c = rho*c/dt.  A FIXED
+// cell has M = 1 and its own row has no couplings (row weight w = 0; w = 1
+// for COMPUTE).  g is a property of the face: the same number from both
+// COMPUTE sides (crank_nicolson.pyx:298-386 derives it per cell).
+//
+// Each cell therefore carries, per axis, an entry (M, w, g_minus).  Two
+// views deliver it:
+//   ClassView: one uint8 class id per cell per axis (built once on the
+//              device from the upstream uint8 maps) indexing a <=256-entry
+//              table held in LDS -- 1 byte per cell per sweep;
+//   DenseView: per-cell float64 arrays M, g_z, g_y, g_x and a uint8 w
+//              (per-voxel parameters, e.g. for gradients).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -15,21 +23,63 @@ namespace hs2 {
 
 constexpr int kWave = 64;
 constexpr int kMaxVtab = 256;
+constexpr int kMaxClass = 256;
+constexpr int kTabStride = 4;  // doubles per class-table entry: M, w, g, pad
 
 struct Grid {
   int nz, ny, nx;
 };
 
-// Compact description: device uint8 maps + a device copy of the tables.
-// Table layout (doubles): g[nb][3][nmat][nmat], then M[nmat], then
-// compute-flag[nmat] (1.0 / 0.0).
-struct Compact {
-  const uint8_t *__restrict__ mat;
-  const uint8_t *__restrict__ bz;
-  const uint8_t *__restrict__ by;
-  const uint8_t *__restrict__ bx;
-  const double *__restrict__ tab;
-  int nmat, nb, tab_len;
+struct Entry {
+  double M, w, g;
+};
+
+// ---------------------------------------------------------------- views
+struct ClassView {
+  const uint8_t *__restrict__ cls[3];  // per axis 0=z, 1=y, 2=x; n each
+  const double *__restrict__ tab;      // global tables, axis a at tab + off[a]
+  int off[4];                          // doubles; off[3] = total length
+  using Raw = uint32_t;
+  // doubles of LDS for the tables of all axes (axis < 0) or one axis
+  __host__ __device__ int lds_doubles(int axis) const {
+    return axis < 0 ? off[3] : off[axis + 1] - off[axis];
+  }
+  // copy all tables (axis < 0) or one axis's table to lds
+  __device__ void load_tables(double *lds, int axis) const {
+    const int a0 = axis < 0 ? 0 : off[axis];
+    const int len = lds_doubles(axis);
+    for (int t = threadIdx.x; t < len; t += blockDim.x) lds[t] = tab[a0 + t];
+  }
+  // table of `axis` inside lds filled by load_tables(lds, -1)
+  __device__ __forceinline__ const double *table(const double *lds, int axis) const {
+    return lds + off[axis];
+  }
+  __device__ __forceinline__ Raw fetch(int axis, int c) const { return cls[axis][c]; }
+  __device__ __forceinline__ Entry resolve(const double *t, Raw r) const {
+    const double2 mw = *reinterpret_cast<const double2 *>(t + r * kTabStride);
+    return Entry{mw.x, mw.y, t[r * kTabStride + 2]};
+  }
+};
+
+struct DenseRaw {
+  double M, g;
+  uint32_t w;
+};
+
+struct DenseView {
+  const double *__restrict__ M;
+  const uint8_t *__restrict__ w;
+  const double *__restrict__ g[3];
+  using Raw = DenseRaw;
+  __host__ __device__ int lds_doubles(int) const { return 0; }
+  __device__ void load_tables(double *, int) const {}
+  __device__ __forceinline__ const double *table(const double *lds, int) const { return lds; }
+  __device__ __forceinline__ Raw fetch(int axis, int c) const {
+    return DenseRaw{M[c], g[axis][c], w[c]};
+  }
+  __device__ __forceinline__ Entry resolve(const double *, Raw r) const {
+    return Entry{r.M, (double)r.w, r.g};
+  }
 };
 
 // Logical block index that keeps consecutive logical blocks on one XCD.
@@ -40,20 +90,55 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
   return x * q + (x < r ? x : r) + idx;
 }
 
-__device__ __forceinline__ void load_table(double *lds, const Compact &cv) {
-  for (int t = threadIdx.x; t < cv.tab_len; t += blockDim.x) lds[t] = cv.tab[t];
+// ------------------------------------------------------- lane exchange
+// Within groups of G lanes (G divides 64).  up(v, s): value of lane l-s of
+// the same group, 0 for the first s lanes; down(v, s): lane l+s, 0 for the
+// last s lanes.  G = 16 uses DPP row shifts (a VALU move, no LDS traffic);
+// wider groups use ds_bpermute.
+template <int G>
+struct LaneX {
+  static __device__ __forceinline__ double up(double v, int s, int lig);
+  static __device__ __forceinline__ double down(double v, int s, int lig);
+};
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-__device__ __forceinline__ double gface(const double *lds, const Compact &cv, int bcls, int axis,
-                                        int mm, int mp) {
-  return lds[((bcls * 3 + axis) * cv.nmat + mm) * cv.nmat + mp];
-}
+// row_shr:s = 0x110+s (lane l reads l-s), row_shl:s = 0x100+s (reads l+s)
+template <>
+struct LaneX<16> {
+  static __device__ __forceinline__ double up(double v, int s, int) {
+    switch (s) {
+      case 1: return dpp_d<0x111>(v);
+      case 2: return dpp_d<0x112>(v);
+      case 4: return dpp_d<0x114>(v);
+      default: return dpp_d<0x118>(v);
+    }
+  }
+  static __device__ __forceinline__ double down(double v, int s, int) {
+    switch (s) {
+      case 1: return dpp_d<0x101>(v);
+      case 2: return dpp_d<0x102>(v);
+      case 4: return dpp_d<0x104>(v);
+      default: return dpp_d<0x108>(v);
+    }
+  }
+};
 
-__device__ __forceinline__ double shfl_up_d(double v, int d) {
-  return __shfl_up(v, (unsigned)d, kWave);
+template <int G>
+__device__ __forceinline__ double LaneX<G>::up(double v, int s, int lig) {
+  const double x = __shfl_up(v, (unsigned)s, kWave);
+  return lig >= s ? x : 0.0;
 }
-__device__ __forceinline__ double shfl_down_d(double v, int d) {
-  return __shfl_down(v, (unsigned)d, kWave);
+template <int G>
+__device__ __forceinline__ double LaneX<G>::down(double v, int s, int lig) {
+  const double x = __shfl_down(v, (unsigned)s, kWave);
+  return lig + s < G ? x : 0.0;
 }
 
 // One PCR reduction of a normalised row  a x_{i-1} + x_i + c x_{i+1} = r
@@ -83,7 +168,6 @@ __device__ __forceinline__ void segment_reduce(double (&a)[N], double (&c)[N], d
                                                double &As, double &Cs, double &Rs, double &Ae,
                                                double &Ce, double &Re) {
   static_assert(N >= 2, "segment needs two rows");
-  // forward: rows 2..N-1 in terms of x_0 (row 1 already is)
 #pragma unroll
   for (int q = 2; q < N; q++) {
     const double k = 1.0 / (1.0 - a[q] * c[q - 1]);
@@ -100,7 +184,6 @@ __device__ __forceinline__ void segment_reduce(double (&a)[N], double (&c)[N], d
     Cs = c[0];
     Rs = r[0];
   } else {
-    // backward: rows N-3..1 in terms of x_0 and x_{N-1} (row N-2 already is)
 #pragma unroll
     for (int q = N - 3; q >= 1; q--) {
       a[q] = a[q] - c[q] * a[q + 1];
@@ -111,6 +194,51 @@ __device__ __forceinline__ void segment_reduce(double (&a)[N], double (&c)[N], d
     As = a[0] * k;
     Cs = -c[0] * c[1] * k;
     Rs = (r[0] - c[0] * r[1]) * k;
+  }
+}
+
+// Solve the G-lane group's tridiagonal system held as CPL normalised rows per
+// lane (lane = consecutive block of CPL unknowns); x receives the solution.
+template <int CPL, int G>
+__device__ __forceinline__ void group_solve(double (&a)[CPL], double (&c)[CPL], double (&r)[CPL],
+                                            double (&x)[CPL], int lig) {
+  using X = LaneX<G>;
+  if constexpr (CPL == 1) {
+    double aa = a[0], cc = c[0], rr = r[0];
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+      const double aL = X::up(aa, d, lig), cL = X::up(cc, d, lig), rL = X::up(rr, d, lig);
+      const double aR = X::down(aa, d, lig), cR = X::down(cc, d, lig), rR = X::down(rr, d, lig);
+      pcr_combine(aa, cc, rr, aL, cL, rL, aR, cR, rR);
+    }
+    x[0] = rr;
+  } else {
+    double As, Cs, Rs, Ae, Ce, Re;
+    segment_reduce<CPL>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
+    // reduced order s0 e0 s1 e1 ...: the first PCR step pairs s_p with
+    // e_{p-1}, e_p and e_p with s_p, s_{p+1}, splitting it into an s- and an
+    // e-system that are then reduced independently across the lanes.
+    {
+      const double aL = X::up(Ae, 1, lig), cL = X::up(Ce, 1, lig), rL = X::up(Re, 1, lig);
+      const double aR = X::down(As, 1, lig), cR = X::down(Cs, 1, lig), rR = X::down(Rs, 1, lig);
+      double as = As, cs = Cs, rs = Rs, ae = Ae, ce = Ce, re = Re;
+      pcr_combine(as, cs, rs, aL, cL, rL, Ae, Ce, Re);
+      pcr_combine(ae, ce, re, As, Cs, Rs, aR, cR, rR);
+      As = as; Cs = cs; Rs = rs; Ae = ae; Ce = ce; Re = re;
+    }
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+      const double aL = X::up(As, d, lig), cL = X::up(Cs, d, lig), rL = X::up(Rs, d, lig);
+      const double aR = X::down(As, d, lig), cR = X::down(Cs, d, lig), rR = X::down(Rs, d, lig);
+      const double eaL = X::up(Ae, d, lig), ecL = X::up(Ce, d, lig), erL = X::up(Re, d, lig);
+      const double eaR = X::down(Ae, d, lig), ecR = X::down(Ce, d, lig), erR = X::down(Re, d, lig);
+      pcr_combine(As, Cs, Rs, aL, cL, rL, aR, cR, rR);
+      pcr_combine(Ae, Ce, Re, eaL, ecL, erL, eaR, ecR, erR);
+    }
+    x[0] = Rs;
+    x[CPL - 1] = Re;
+#pragma unroll
+    for (int q = 1; q < CPL - 1; q++) x[q] = r[q] - a[q] * Rs - c[q] * Re;
   }
 }
 

@@ -6,21 +6,40 @@
 
 namespace hs2 {
 
-// cells per lane for an x-line of nx cells (0 = unsupported, nx > 1024)
-int x_cpl_for(int nx);
+struct XConfig {  // x sweep: CPL cells per lane, G lanes per line
+  int cpl, g;
+};
+struct SConfig {  // strided sweep: SEG cells per thread, CW lines per block
+  int seg, cw;
+};
 
-hipError_t launch_sweep_x(int cpl, const Grid &g, const Compact &cv, const double *Tn, double *Ts,
-                          const uint8_t *vmap, const double *vtab, const double *sfield,
-                          hipStream_t s);
+XConfig x_config_for(int nx);  // {0,0}: unsupported (nx > 1024)
 
-size_t strided_lds_bytes(const Compact &cv, int P, int CW);
+hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const ClassView &v, const double *Tn,
+                          double *Ts, const uint8_t *vmap, const double *vtab,
+                          const double *sfield, hipStream_t s);
+hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const DenseView &v, const double *Tn,
+                          double *Ts, const uint8_t *vmap, const double *vtab,
+                          const double *sfield, hipStream_t s);
 
-hipError_t launch_sweep_strided(int seg, int cw, const Grid &g, const Compact &cv, int axis,
+size_t strided_lds_bytes(const ClassView &v, int axis, int P, int CW);
+size_t strided_lds_bytes(const DenseView &v, int axis, int P, int CW);
+
+hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const ClassView &v, int axis,
+                                const double *Tn, const double *Y, double *X, double *surf,
+                                double dz, hipStream_t s);
+hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const DenseView &v, int axis,
                                 const double *Tn, const double *Y, double *X, double *surf,
                                 double dz, hipStream_t s);
 
-hipError_t launch_expand(const Grid &g, const Compact &cv, double *gz, double *gy, double *gx,
-                         double *M, uint8_t *cflag, hipStream_t s);
+hipError_t launch_classify(const Grid &g, const uint8_t *mat, const uint8_t *bz, const uint8_t *by,
+                           const uint8_t *bx, const uint8_t *lut, const uint8_t *edge, int nmat,
+                           int lutlen, uint8_t *cz, uint8_t *cy, uint8_t *cx, hipStream_t s);
+
+hipError_t launch_expand(const Grid &g, const ClassView &v, double *M, double *w, double *gz,
+                         double *gy, double *gx, hipStream_t s);
+hipError_t launch_expand(const Grid &g, const DenseView &v, double *M, double *w, double *gz,
+                         double *gy, double *gx, hipStream_t s);
 
 hipError_t launch_surface(const double *T, int ny, int nx, double dz, double *out, hipStream_t s);
 

@@ -71,13 +71,35 @@ int hs2_plan_set_compact(hs2_plan *plan, const uint8_t *d_mat, const uint8_t *d_
                          const uint8_t *d_by, const uint8_t *d_bx, int nmat, int nb,
                          const double *h_gtab, const double *h_mtab, const uint8_t *h_compute);
 
-/* Expand the plan's compact coefficients into per-face float64 arrays on the
- * device (d_gz (nz+1,ny,nx), d_gy (nz,ny+1,nx), d_gx (nz,ny,nx+1), d_M (n),
- * d_cflag (n, uint8)); any pointer may be NULL.  Exposes the numbers the
- * upstream keeps in Amat/Bmat (alternatingdirection_c_pyx.pyx:230,251) for
- * inspection. */
-int hs2_plan_expand(hs2_plan *plan, double *d_gz, double *d_gy, double *d_gx, double *d_M,
-                    uint8_t *d_cflag, void *stream);
+/* Per-voxel coefficients instead of the maps: caller-owned device arrays of
+ * n entries, cell-indexed in C order -- d_M: rho*c/dt (1 for a FIXED cell);
+ * d_w: 1 COMPUTE / 0 FIXED (uint8); d_gz, d_gy, d_gx: conductance per
+ * volume of each cell's minus face on that axis (the face towards k-1, j-1,
+ * i-1; 0 on the outer faces).  The plus face of a cell is the minus face of
+ * its neighbour.  Same numbers as hs2_plan_set_compact derives, for
+ * per-voxel material fields that no uint8 map can express; the upstream has
+ * no per-voxel mode (its maps cap materials at 256, crank_nicolson.pyx:135). */
+int hs2_plan_set_dense(hs2_plan *plan, const double *d_M, const uint8_t *d_w, const double *d_gz,
+                       const double *d_gy, const double *d_gx);
+
+/* Write the plan's per-cell coefficients as float64 arrays (n each; any
+ * pointer may be NULL): M, w, and the minus-face conductances per axis --
+ * the numbers the upstream keeps in Amat/Bmat
+ * (alternatingdirection_c_pyx.pyx:230,251), for inspection. */
+int hs2_plan_expand(hs2_plan *plan, double *d_M, double *d_w, double *d_gz, double *d_gy,
+                    double *d_gx, void *stream);
+
+/* Kernel launch shapes (used by the autotuner): the x sweep runs x_lanes
+ * lanes of x_cpl consecutive cells per x-line (x_cpl*x_lanes >= nx;
+ * x_lanes in {16,32,64}, x_cpl in {1,2,4,8,16}); the y and z sweeps split
+ * each line into segments of *_seg cells (one thread each) and put *_lines
+ * lines in a block (segments*lines <= 512).  No upstream counterpart. */
+int hs2_plan_set_config(hs2_plan *plan, int x_cpl, int x_lanes, int y_seg, int y_lines,
+                        int z_seg, int z_lines);
+
+/* info[0..9] = mode (0 none, 1 class maps, 2 dense), classes per axis z/y/x,
+ * x_cpl, x_lanes, y_seg, y_lines, z_seg, z_lines. */
+int hs2_plan_info(const hs2_plan *plan, int *info);
 
 /* One full Douglas step T^n -> T^{n+1} (x, y, z substeps).
  *   d_Tin, d_Tout: n doubles (may alias: in-place is supported);

@@ -84,28 +84,31 @@ def test_steel_cube_history_matches_upstream(hs):
 
 
 def test_expand_matches_oracle_coefficients(hs):
-    """The device-expanded per-face conductances and M equal the oracle's
-    per-cell restatement of crank_nicolson.pyx."""
+    """The device-built per-cell coefficients (classes from the maps) equal the
+    oracle's per-cell restatement of crank_nicolson.pyx."""
     from heatsim2_amd import _lib
     for name in ("tiny_mixed", "tiny_aniso", "laminate_mini"):
         case = load_golden(name)
         args = to_package_args(hs, case)
         P, S = hs.setup(*args["setup"])
+        assert _lib.plan_info(P.plan)["mode"] == 1
         nz, ny, nx = case["shape"]
-        dev = torch.device("cuda")
-        gz = torch.zeros((nz + 1, ny, nx), dtype=torch.float64, device=dev)
-        gy = torch.zeros((nz, ny + 1, nx), dtype=torch.float64, device=dev)
-        gx = torch.zeros((nz, ny, nx + 1), dtype=torch.float64, device=dev)
-        M = torch.zeros((nz, ny, nx), dtype=torch.float64, device=dev)
-        cf = torch.zeros((nz, ny, nx), dtype=torch.uint8, device=dev)
+        f = [torch.zeros((nz, ny, nx), dtype=torch.float64, device="cuda") for _ in range(5)]
         _lib.check("hs2_plan_expand", _lib.load().hs2_plan_expand(
-            P.plan, _lib.ptr(gz), _lib.ptr(gy), _lib.ptr(gx), _lib.ptr(M), _lib.ptr(cf),
-            _lib.stream_handle()))
-        gz, gy, gx, M, cf = [t.cpu().numpy() for t in (gz, gy, gx, M, cf)]
+            P.plan, *[_lib.ptr(t) for t in f], _lib.stream_handle()))
+        M, w, gz, gy, gx = [t.cpu().numpy() for t in f]
         ref = oracle.assemble_case(case).reshape(nz, ny, nx, 8)
-        c = cf.astype(float)
-        got = np.stack([M, c, gz[:-1] * c, gz[1:] * c, gy[:, :-1] * c, gy[:, 1:] * c,
-                        gx[:, :, :-1] * c, gx[:, :, 1:] * c], axis=-1)
+
+        def plus(g, axis):
+            out = np.zeros_like(g)
+            sl = [slice(None)] * 3
+            sl2 = [slice(None)] * 3
+            sl[axis] = slice(0, -1)
+            sl2[axis] = slice(1, None)
+            out[tuple(sl)] = g[tuple(sl2)]
+            return out
+        got = np.stack([M, w, gz * w, plus(gz, 0) * w, gy * w, plus(gy, 1) * w, gx * w,
+                        plus(gx, 2) * w], axis=-1)
         assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)) < 1e-15, name
 
 
@@ -198,3 +201,33 @@ def test_deterministic(hs):
                                       args["vmap"], args["volumetric"])
         outs.append(T)
     assert torch.equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("shape", [(20, 48, 200), (130, 9, 40), (9, 300, 12)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_every_launch_shape_matches_oracle(hs, shape):
+    """Every launch shape the autotuner may pick (x: cells per lane x lanes per
+    line; y, z: segment length x lines per block) gives the oracle's result."""
+    from heatsim2_amd import _lib, autotune
+    case = _random_case(shape, seed=7 + sum(shape))
+    case["nsteps"] = 2
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    ref = oracle.run_ref(case)
+    rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
+    info = _lib.plan_info(P.plan)
+    base = {"x": (info["x_cpl"], info["x_lanes"]), "y": (info["y_seg"], info["y_lines"]),
+            "z": (info["z_seg"], info["z_lines"])}
+    nz, ny, nx = shape
+    trials = [("x", c) for c in autotune.x_candidates(nx)] + \
+             [("y", c) for c in autotune.strided_candidates(ny)] + \
+             [("z", c) for c in autotune.strided_candidates(nz)]
+    for axis, cand in trials:
+        cfg = dict(base)
+        cfg[axis] = cand
+        autotune._apply(_lib.load(), P.plan, cfg)
+        T = torch.from_numpy(case["T0"]).cuda()
+        for n in range(case["nsteps"]):
+            T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"],
+                                 args["volumetric"])
+            assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, (axis, cand, n)
