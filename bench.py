@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, nargs=3, default=(256, 256, 256))
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--config", default=None,
+                    help="force launch shapes, e.g. 'x=4,64,0;y=16,16;z=16,16' (skips the tuner)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256^3 float64 field 20 times (scales HBM counters)")
@@ -107,7 +109,13 @@ def main():
     n = nz * ny * nx
     dt = case["dt"]
     T = torch.from_numpy(case["T0"]).to(dev)
-    if not args.no_tune:
+    if args.config:
+        cfg = {}
+        for part in args.config.split(";"):
+            k, v = part.split("=")
+            cfg[k.strip()] = [int(t) for t in v.split(",")]
+        hs.autotune._apply(_lib.load(), P.plan, cfg)
+    elif not args.no_tune:
         t0 = time.time()
         cfg = hs.autotune.tune(P, T, use_cache=False, verbose=args.verbose)
         log("autotune %.2f s: %s" % (time.time() - t0, cfg))
@@ -154,8 +162,8 @@ def main():
                                   "layer h=250 W/m^2K, flash %.1f J/m^2, dt=0.5 ms, FP64, surface "
                                   "readout every step" % (nz, ny, nx, case["flash_J_m2"]),
                       "cells": n, "parallelism": "single GPU",
-                      "launch": {k: tuned[k] for k in ("x_cpl", "x_lanes", "y_seg", "y_lines",
-                                                       "z_seg", "z_lines", "mode")}},
+                      "launch": {k: tuned[k] for k in ("x_cpl", "x_lanes", "x_lines", "y_seg",
+                                                       "y_lines", "z_seg", "z_lines", "mode")}},
            "roofline": roofline}
     del T, hist
     if not args.no_cpu:

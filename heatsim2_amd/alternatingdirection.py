@@ -83,6 +83,8 @@ def _as_device_field(Tarray, shape):
         if Tarray.dtype != torch.float64:
             raise ValueError("Tarray must be float64")
         T = Tarray.to(dev).contiguous()
+        if T.data_ptr() % 16:
+            T = T.clone()  # the kernels use 16-byte vector loads
         return T, True
     a = np.ascontiguousarray(Tarray, dtype=np.float64)
     if a.shape != tuple(shape):

@@ -20,8 +20,12 @@ import torch
 
 from heatsim2_amd import _lib
 
-X_COMBOS = ((1, 16), (2, 16), (4, 16), (8, 16), (16, 16), (2, 32), (4, 32), (8, 32), (16, 32),
-            (2, 64), (4, 64), (8, 64), (16, 64))
+# (cells per lane, lanes per line, lines per block of the two-phase kernel or 0)
+X_COMBOS = ((1, 16, 0), (2, 16, 0), (4, 16, 0), (8, 16, 0), (16, 16, 0), (2, 32, 0), (4, 32, 0),
+            (8, 32, 0), (16, 32, 0), (2, 64, 0), (4, 64, 0), (8, 64, 0), (16, 64, 0),
+            (1, 16, 8), (2, 16, 8), (4, 16, 8), (8, 16, 8), (16, 16, 8), (4, 16, 16),
+            (8, 16, 16), (16, 16, 4), (16, 16, 16), (8, 32, 4), (16, 32, 4), (16, 32, 2),
+            (16, 64, 2), (8, 64, 2))
 SEGS = (2, 4, 8, 16, 32)
 LINES = (8, 16, 32, 64)
 MAX_BLOCK = 512
@@ -80,23 +84,24 @@ def _time(L, plan, T, steps):
 
 def tune(ADI_params, T=None, steps=4, use_cache=True, verbose=False):
     """Pick and set the fastest launch shapes for this plan; returns
-    {"x": (cpl, lanes), "y": (seg, lines), "z": (seg, lines), "ms": [x, y, z]}."""
+    {"x": (cpl, lanes, lines), "y": (seg, lines), "z": (seg, lines), "ms": [x, y, z]}."""
     L = _lib.load()
     plan = ADI_params.plan
     nz, ny, nx = ADI_params.shape
     info = _lib.plan_info(plan)
-    key = "%s|%dx%dx%d|mode%d" % (torch.cuda.get_device_name(), nz, ny, nx, info["mode"])
+    key = "%s|%s|%dx%dx%d|mode%d" % (L.hs2_version().decode(), torch.cuda.get_device_name(),
+                                     nz, ny, nx, info["mode"])
     if use_cache:
         cfg = _memo.get(key) or _load_cache().get(key)
-        if cfg:
-            _apply(L, plan, cfg)
+        if cfg and _apply(L, plan, cfg, strict=False):
             _memo[key] = cfg
             return cfg
     dev = _lib.device()
     if T is None:
         T = torch.full(ADI_params.shape, 293.0, dtype=torch.float64, device=dev)
     scratch = T.detach().clone()
-    cur = {"x": (info["x_cpl"], info["x_lanes"]), "y": (info["y_seg"], info["y_lines"]),
+    cur = {"x": (info["x_cpl"], info["x_lanes"], info["x_lines"]),
+           "y": (info["y_seg"], info["y_lines"]),
            "z": (info["z_seg"], info["z_lines"])}
     best_ms = [None, None, None]
     for axis_name, idx, cands in (("x", 0, x_candidates(nx)), ("y", 1, strided_candidates(ny)),
@@ -126,8 +131,9 @@ def tune(ADI_params, T=None, steps=4, use_cache=True, verbose=False):
 
 
 def _apply(L, plan, cfg, strict=True):
-    rc = L.hs2_plan_set_config(plan, int(cfg["x"][0]), int(cfg["x"][1]), int(cfg["y"][0]),
-                               int(cfg["y"][1]), int(cfg["z"][0]), int(cfg["z"][1]))
+    rc = L.hs2_plan_set_config(plan, int(cfg["x"][0]), int(cfg["x"][1]), int(cfg["x"][2]),
+                               int(cfg["y"][0]), int(cfg["y"][1]), int(cfg["z"][0]),
+                               int(cfg["z"][1]))
     if strict:
         _lib.check("hs2_plan_set_config", rc)
     return rc == 0

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -58,7 +59,7 @@ struct hs2_plan {
   double *d_vsteps = nullptr;   // per-step source tables (hs2_run)
   size_t vsteps_cap = 0;        // doubles
   int *d_check = nullptr;       // per-block maxima of the maps (validation)
-  hs2::XConfig xcfg{0, 0};
+  hs2::XConfig xcfg{0, 0, 0};
   hs2::SConfig scfg[2] = {{0, 0}, {0, 0}};  // [0] z, [1] y
 };
 
@@ -89,11 +90,13 @@ bool strided_ok(int L, hs2::SConfig c) {
 }
 
 bool x_ok(int nx, hs2::XConfig c) {
-  static const int combos[][2] = {{1, 16}, {2, 16}, {4, 16}, {8, 16}, {16, 16},
-                                  {2, 32}, {4, 32}, {8, 32}, {16, 32},
-                                  {2, 64}, {4, 64}, {8, 64}, {16, 64}};
+  static const int combos[][3] = {
+      {1, 16, 0}, {2, 16, 0}, {4, 16, 0}, {8, 16, 0}, {16, 16, 0}, {2, 32, 0}, {4, 32, 0},
+      {8, 32, 0}, {16, 32, 0}, {2, 64, 0}, {4, 64, 0}, {8, 64, 0}, {16, 64, 0},
+      {1, 16, 8}, {2, 16, 8}, {4, 16, 8}, {8, 16, 8}, {16, 16, 8}, {4, 16, 16}, {8, 16, 16},
+      {16, 16, 4}, {16, 16, 16}, {8, 32, 4}, {16, 32, 4}, {16, 32, 2}, {16, 64, 2}, {8, 64, 2}};
   for (auto &cb : combos)
-    if (cb[0] == c.cpl && cb[1] == c.g) return c.cpl * c.g >= nx;
+    if (cb[0] == c.cpl && cb[1] == c.g && cb[2] == c.lpb) return c.cpl * c.g >= nx;
   return false;
 }
 
@@ -147,6 +150,8 @@ int map_max(const uint8_t *d_map, size_t n, int *d_scratch, int *result) {
   *result = *std::max_element(part, part + kCheckBlocks);
   return HS2_OK;
 }
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 hs2::Grid grid_of(const hs2_plan *p) { return hs2::Grid{p->grid.nz, p->grid.ny, p->grid.nx}; }
 
@@ -374,6 +379,9 @@ int hs2_plan_set_dense(hs2_plan *p, const double *d_M, const uint8_t *d_w, const
                        const double *d_gy, const double *d_gx) {
   if (!p || !d_M || !d_w || !d_gz || !d_gy || !d_gx)
     return fail(HS2_E_ARG, "hs2_plan_set_dense: null argument");
+  if (!aligned16(d_M) || !aligned16(d_w) || !aligned16(d_gz) || !aligned16(d_gy) ||
+      !aligned16(d_gx))
+    return fail(HS2_E_ARG, "hs2_plan_set_dense: arrays must be 16-byte aligned");
   free_coefficients(p);
   p->dv.M = d_M;
   p->dv.w = d_w;
@@ -386,6 +394,7 @@ int hs2_plan_set_dense(hs2_plan *p, const double *d_M, const uint8_t *d_w, const
 
 int hs2_plan_info(const hs2_plan *p, int *info) {
   if (!p || !info) return fail(HS2_E_ARG, "hs2_plan_info: null argument");
+  info[10] = p->xcfg.lpb;
   info[0] = p->mode;
   info[1] = p->ncls[0];
   info[2] = p->ncls[1];
@@ -399,14 +408,14 @@ int hs2_plan_info(const hs2_plan *p, int *info) {
   return HS2_OK;
 }
 
-int hs2_plan_set_config(hs2_plan *p, int x_cpl, int x_lanes, int y_seg, int y_lines, int z_seg,
-                        int z_lines) {
+int hs2_plan_set_config(hs2_plan *p, int x_cpl, int x_lanes, int x_lines, int y_seg, int y_lines,
+                        int z_seg, int z_lines) {
   if (!p) return fail(HS2_E_ARG, "hs2_plan_set_config: null plan");
-  const hs2::XConfig xc{x_cpl, x_lanes};
+  const hs2::XConfig xc{x_cpl, x_lanes, x_lines};
   const hs2::SConfig yc{y_seg, y_lines}, zc{z_seg, z_lines};
   if (!x_ok(p->grid.nx, xc))
-    return fail(HS2_E_ARG, "hs2_plan_set_config: x config %dx%d invalid for nx=%d", x_cpl, x_lanes,
-                p->grid.nx);
+    return fail(HS2_E_ARG, "hs2_plan_set_config: x config %dx%dx%d invalid for nx=%d", x_cpl,
+                x_lanes, x_lines, p->grid.nx);
   if (!strided_ok(p->grid.ny, yc))
     return fail(HS2_E_ARG, "hs2_plan_set_config: y config seg=%d lines=%d invalid for ny=%d",
                 y_seg, y_lines, p->grid.ny);
@@ -435,6 +444,8 @@ int hs2_step(hs2_plan *p, const double *d_Tin, double *d_Tout, const uint8_t *d_
              const double *h_vtab, int nvtab, const double *d_sfield, double *d_surf,
              void *stream) {
   if (!p || !d_Tin || !d_Tout) return fail(HS2_E_ARG, "hs2_step: null argument");
+  if (!aligned16(d_Tin) || !aligned16(d_Tout) || (d_sfield && !aligned16(d_sfield)))
+    return fail(HS2_E_ARG, "hs2_step: field pointers must be 16-byte aligned");
   if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_step: coefficients not set");
   if (d_surf && p->grid.nz < 2) return fail(HS2_E_ARG, "hs2_step: surface readout needs nz >= 2");
   if ((d_vmap != nullptr) != (h_vtab != nullptr))
@@ -456,6 +467,7 @@ int hs2_step(hs2_plan *p, const double *d_Tin, double *d_Tout, const uint8_t *d_
 int hs2_run(hs2_plan *p, double *d_T, int nsteps, const uint8_t *d_vmap,
             const double *h_vtab_steps, int nvtab, double *d_surf_hist, void *stream) {
   if (!p || !d_T) return fail(HS2_E_ARG, "hs2_run: null argument");
+  if (!aligned16(d_T)) return fail(HS2_E_ARG, "hs2_run: field pointer must be 16-byte aligned");
   if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_run: coefficients not set");
   if (nsteps < 0) return fail(HS2_E_ARG, "hs2_run: nsteps=%d", nsteps);
   if (d_surf_hist && p->grid.nz < 2) return fail(HS2_E_ARG, "hs2_run: surface readout needs nz >= 2");
@@ -499,7 +511,8 @@ int hs2_run(hs2_plan *p, double *d_T, int nsteps, const uint8_t *d_vmap,
 }
 
 int hs2_run_timed(hs2_plan *p, double *d_T, int nsteps, double *kernel_ms, void *stream) {
-  if (!p || !d_T || !kernel_ms || nsteps < 1) return fail(HS2_E_ARG, "hs2_run_timed: bad argument");
+  if (!p || !d_T || !kernel_ms || nsteps < 1 || !aligned16(d_T))
+    return fail(HS2_E_ARG, "hs2_run_timed: bad argument");
   if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_run_timed: coefficients not set");
   hipStream_t s = (hipStream_t)stream;
   std::vector<hipEvent_t> ev((size_t)nsteps * 4, nullptr);

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace hs2 {
 
 constexpr int kWave = 64;
@@ -55,6 +57,14 @@ struct ClassView {
     return lds + off[axis];
   }
   __device__ __forceinline__ Raw fetch(int axis, int c) const { return cls[axis][c]; }
+  // four consecutive cells starting at c (c % 4 == 0)
+  __device__ __forceinline__ void fetch4(int axis, int c, Raw (&r)[4]) const {
+    const uint32_t w = *reinterpret_cast<const uint32_t *>(cls[axis] + c);
+    r[0] = w & 0xffu;
+    r[1] = (w >> 8) & 0xffu;
+    r[2] = (w >> 16) & 0xffu;
+    r[3] = w >> 24;
+  }
   __device__ __forceinline__ Entry resolve(const double *t, Raw r) const {
     const double2 mw = *reinterpret_cast<const double2 *>(t + r * kTabStride);
     return Entry{mw.x, mw.y, t[r * kTabStride + 2]};
@@ -76,6 +86,17 @@ struct DenseView {
   __device__ __forceinline__ const double *table(const double *lds, int) const { return lds; }
   __device__ __forceinline__ Raw fetch(int axis, int c) const {
     return DenseRaw{M[c], g[axis][c], w[c]};
+  }
+  __device__ __forceinline__ void fetch4(int axis, int c, Raw (&r)[4]) const {
+    const double2 m01 = *reinterpret_cast<const double2 *>(M + c);
+    const double2 m23 = *reinterpret_cast<const double2 *>(M + c + 2);
+    const double2 g01 = *reinterpret_cast<const double2 *>(g[axis] + c);
+    const double2 g23 = *reinterpret_cast<const double2 *>(g[axis] + c + 2);
+    const uint32_t ww = *reinterpret_cast<const uint32_t *>(w + c);
+    r[0] = DenseRaw{m01.x, g01.x, ww & 0xffu};
+    r[1] = DenseRaw{m01.y, g01.y, (ww >> 8) & 0xffu};
+    r[2] = DenseRaw{m23.x, g23.x, (ww >> 16) & 0xffu};
+    r[3] = DenseRaw{m23.y, g23.y, ww >> 24};
   }
   __device__ __forceinline__ Entry resolve(const double *, Raw r) const {
     return Entry{r.M, (double)r.w, r.g};
@@ -141,13 +162,26 @@ __device__ __forceinline__ double LaneX<G>::down(double v, int s, int lig) {
   return lig + s < G ? x : 0.0;
 }
 
+// 1/x from the hardware reciprocal refined by one Newton step (error within a
+// couple of ulp of the correctly rounded quotient; x is a well-scaled
+// positive pivot here, never 0, inf or denormal).
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  const double e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// Couplings below this magnitude cannot change an O(1)-relative solution by
+// even one ulp (2^-64 << 2^-53), so the reduced system is already diagonal.
+constexpr double kNegligible = 5.421010862427522e-20;  // 2^-64
+
 // One PCR reduction of a normalised row  a x_{i-1} + x_i + c x_{i+1} = r
 // with its neighbour rows L (i-1) and R (i+1), giving a row that couples
 // i-2 and i+2.  Absent neighbours are passed as zeros.
 __device__ __forceinline__ void pcr_combine(double &a, double &c, double &r, double aL,
                                             double cL, double rL, double aR, double cR,
                                             double rR) {
-  const double k = 1.0 / (1.0 - a * cL - c * aR);
+  const double k = rcp(1.0 - a * cL - c * aR);
   const double an = -a * aL * k;
   const double cn = -c * cR * k;
   const double rn = (r - a * rL - c * rR) * k;
@@ -156,13 +190,14 @@ __device__ __forceinline__ void pcr_combine(double &a, double &c, double &r, dou
   r = rn;
 }
 
-// Segment elimination.  Rows q = 0..N-1 of one contiguous segment, already
-// normalised (diagonal 1): a[q] x_{q-1} + x_q + c[q] x_{q+1} = r[q].
-// Afterwards:
+// Segment elimination of rows already normalised to unit diagonal,
+// a_q x_{q-1} + x_q + c_q x_{q+1} = r_q, q = 0..N-1 of one contiguous segment.
+// Leaves the two end rows
 //   (As, Cs, Rs): As x_{before} + x_0     + Cs x_{N-1}  = Rs
 //   (Ae, Ce, Re): Ae x_0        + x_{N-1} + Ce x_{after} = Re
-// and, for interior rows 1..N-2, a[q] x_0 + x_q + c[q] x_{N-1} = r[q]
-// (a, c, r overwritten).  N >= 2.
+// and, in place for interior rows 1..N-2, a[q] x_0 + x_q + c[q] x_{N-1} = r[q].
+// Forward pass eliminates x_{q-1} from rows 2..N-1 (expressing them in x_0),
+// backward pass eliminates x_{q+1} from rows N-3..1 (in x_{N-1}).  N >= 2.
 template <int N>
 __device__ __forceinline__ void segment_reduce(double (&a)[N], double (&c)[N], double (&r)[N],
                                                double &As, double &Cs, double &Rs, double &Ae,
@@ -197,48 +232,132 @@ __device__ __forceinline__ void segment_reduce(double (&a)[N], double (&c)[N], d
   }
 }
 
-// Solve the G-lane group's tridiagonal system held as CPL normalised rows per
-// lane (lane = consecutive block of CPL unknowns); x receives the solution.
+// Segment elimination, streamed row by row.  Rows q = 0..N-1 of one
+// contiguous segment, a_q x_{q-1} + b_q x_q + c_q x_{q+1} = r_q (b_q > 0),
+// are fed in order with row(q, ...); one reciprocal per row.  finish() then
+// leaves
+//   (As, Cs, Rs): As x_{before} + x_0     + Cs x_{N-1}  = Rs
+//   (Ae, Ce, Re): Ae x_0        + x_{N-1} + Ce x_{after} = Re
+// and, for interior rows 1..N-2, a[q] x_0 + x_q + c[q] x_{N-1} = r[q].
+// N >= 2.
+template <int N>
+struct Seg {
+  static_assert(N >= 2, "segment needs two rows");
+  double a[N], c[N], r[N];
+  double b0;
+  __device__ __forceinline__ void row(int q, double aq, double bq, double cq, double rq) {
+    if (q == 0) {
+      a[0] = aq;
+      b0 = bq;
+      c[0] = cq;
+      r[0] = rq;
+    } else if (q == 1) {
+      const double k = rcp(bq);
+      a[1] = aq * k;
+      c[1] = cq * k;
+      r[1] = rq * k;
+    } else {
+      const double k = rcp(bq - aq * c[q - 1]);
+      a[q] = -aq * a[q - 1] * k;
+      c[q] = cq * k;
+      r[q] = (rq - aq * r[q - 1]) * k;
+    }
+  }
+  __device__ __forceinline__ void finish(double &As, double &Cs, double &Rs, double &Ae,
+                                         double &Ce, double &Re) {
+    Ae = a[N - 1];
+    Ce = c[N - 1];
+    Re = r[N - 1];
+    if constexpr (N == 2) {
+      const double k = rcp(b0);
+      As = a[0] * k;
+      Cs = c[0] * k;
+      Rs = r[0] * k;
+    } else {
+#pragma unroll
+      for (int q = N - 3; q >= 1; q--) {
+        a[q] = a[q] - c[q] * a[q + 1];
+        r[q] = r[q] - c[q] * r[q + 1];
+        c[q] = -c[q] * c[q + 1];
+      }
+      const double k = rcp(b0 - c[0] * a[1]);
+      As = a[0] * k;
+      Cs = -c[0] * c[1] * k;
+      Rs = (r[0] - c[0] * r[1]) * k;
+    }
+  }
+  // interior solution once the segment's end values are known
+  __device__ __forceinline__ double interior(int q, double xs, double xe) const {
+    return r[q] - a[q] * xs - c[q] * xe;
+  }
+};
+
+__device__ __forceinline__ double coupling(double As, double Cs, double Ae, double Ce) {
+  return fmax(fmax(fabs(As), fabs(Cs)), fmax(fabs(Ae), fabs(Ce)));
+}
+
+// Single-row "segment" (CPL = 1): the row itself, normalised.
+struct Row1 {
+  double a, c, r;
+  __device__ __forceinline__ void row(int, double aq, double bq, double cq, double rq) {
+    const double k = rcp(bq);
+    a = aq * k;
+    c = cq * k;
+    r = rq * k;
+  }
+};
+
+template <int CPL>
+using SegOf = typename std::conditional<CPL == 1, Row1, Seg<CPL>>::type;
+
+// Solve the G-lane group's tridiagonal system: each lane holds CPL
+// consecutive unknowns whose rows were fed to `s`; x receives the solution.
+// The cross-lane reduction is skipped when every coupling left after the
+// in-lane elimination is negligible for the whole wavefront (a
+// wave-uniform decision).
 template <int CPL, int G>
-__device__ __forceinline__ void group_solve(double (&a)[CPL], double (&c)[CPL], double (&r)[CPL],
-                                            double (&x)[CPL], int lig) {
+__device__ __forceinline__ void group_solve(SegOf<CPL> &s, double (&x)[CPL], int lig) {
   using X = LaneX<G>;
   if constexpr (CPL == 1) {
-    double aa = a[0], cc = c[0], rr = r[0];
+    double aa = s.a, cc = s.c, rr = s.r;
+    if (__any(fmax(fabs(aa), fabs(cc)) > kNegligible)) {
 #pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-      const double aL = X::up(aa, d, lig), cL = X::up(cc, d, lig), rL = X::up(rr, d, lig);
-      const double aR = X::down(aa, d, lig), cR = X::down(cc, d, lig), rR = X::down(rr, d, lig);
-      pcr_combine(aa, cc, rr, aL, cL, rL, aR, cR, rR);
+      for (int d = 1; d < G; d <<= 1) {
+        const double aL = X::up(aa, d, lig), cL = X::up(cc, d, lig), rL = X::up(rr, d, lig);
+        const double aR = X::down(aa, d, lig), cR = X::down(cc, d, lig), rR = X::down(rr, d, lig);
+        pcr_combine(aa, cc, rr, aL, cL, rL, aR, cR, rR);
+      }
     }
     x[0] = rr;
   } else {
     double As, Cs, Rs, Ae, Ce, Re;
-    segment_reduce<CPL>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
-    // reduced order s0 e0 s1 e1 ...: the first PCR step pairs s_p with
-    // e_{p-1}, e_p and e_p with s_p, s_{p+1}, splitting it into an s- and an
-    // e-system that are then reduced independently across the lanes.
-    {
-      const double aL = X::up(Ae, 1, lig), cL = X::up(Ce, 1, lig), rL = X::up(Re, 1, lig);
-      const double aR = X::down(As, 1, lig), cR = X::down(Cs, 1, lig), rR = X::down(Rs, 1, lig);
-      double as = As, cs = Cs, rs = Rs, ae = Ae, ce = Ce, re = Re;
-      pcr_combine(as, cs, rs, aL, cL, rL, Ae, Ce, Re);
-      pcr_combine(ae, ce, re, As, Cs, Rs, aR, cR, rR);
-      As = as; Cs = cs; Rs = rs; Ae = ae; Ce = ce; Re = re;
-    }
+    s.finish(As, Cs, Rs, Ae, Ce, Re);
+    if (__any(coupling(As, Cs, Ae, Ce) > kNegligible)) {
+      // reduced order s0 e0 s1 e1 ...: the first PCR step pairs s_p with
+      // e_{p-1}, e_p and e_p with s_p, s_{p+1}, splitting it into an s- and
+      // an e-system that are then reduced independently across the lanes.
+      {
+        const double aL = X::up(Ae, 1, lig), cL = X::up(Ce, 1, lig), rL = X::up(Re, 1, lig);
+        const double aR = X::down(As, 1, lig), cR = X::down(Cs, 1, lig), rR = X::down(Rs, 1, lig);
+        double as = As, cs = Cs, rs = Rs, ae = Ae, ce = Ce, re = Re;
+        pcr_combine(as, cs, rs, aL, cL, rL, Ae, Ce, Re);
+        pcr_combine(ae, ce, re, As, Cs, Rs, aR, cR, rR);
+        As = as; Cs = cs; Rs = rs; Ae = ae; Ce = ce; Re = re;
+      }
 #pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-      const double aL = X::up(As, d, lig), cL = X::up(Cs, d, lig), rL = X::up(Rs, d, lig);
-      const double aR = X::down(As, d, lig), cR = X::down(Cs, d, lig), rR = X::down(Rs, d, lig);
-      const double eaL = X::up(Ae, d, lig), ecL = X::up(Ce, d, lig), erL = X::up(Re, d, lig);
-      const double eaR = X::down(Ae, d, lig), ecR = X::down(Ce, d, lig), erR = X::down(Re, d, lig);
-      pcr_combine(As, Cs, Rs, aL, cL, rL, aR, cR, rR);
-      pcr_combine(Ae, Ce, Re, eaL, ecL, erL, eaR, ecR, erR);
+      for (int d = 1; d < G; d <<= 1) {
+        const double aL = X::up(As, d, lig), cL = X::up(Cs, d, lig), rL = X::up(Rs, d, lig);
+        const double aR = X::down(As, d, lig), cR = X::down(Cs, d, lig), rR = X::down(Rs, d, lig);
+        const double eaL = X::up(Ae, d, lig), ecL = X::up(Ce, d, lig), erL = X::up(Re, d, lig);
+        const double eaR = X::down(Ae, d, lig), ecR = X::down(Ce, d, lig), erR = X::down(Re, d, lig);
+        pcr_combine(As, Cs, Rs, aL, cL, rL, aR, cR, rR);
+        pcr_combine(Ae, Ce, Re, eaL, ecL, erL, eaR, ecR, erR);
+      }
     }
     x[0] = Rs;
     x[CPL - 1] = Re;
 #pragma unroll
-    for (int q = 1; q < CPL - 1; q++) x[q] = r[q] - a[q] * Rs - c[q] * Re;
+    for (int q = 1; q < CPL - 1; q++) x[q] = s.interior(q, Rs, Re);
   }
 }
 

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
   ex[CPL].g = LaneX<G>::down(ex[0].g, 1, lig);
   const double TL = LaneX<G>::up(T[CPL - 1], 1, lig);
   const double TR = LaneX<G>::down(T[0], 1, lig);
-  double a[CPL], c[CPL], r[CPL];
+  SegOf<CPL> seg;
 #pragma unroll
   for (int q = 0; q < CPL; q++) {
     const int i = i0 + q;
@@ -112,17 +112,280 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
                         gyp * (Typ[q] - Tc) + gzm * (Tzm[q] - Tc) + gzp * (Tzp[q] - Tc) + S[q];
     const double rhs = M * Tc + w * expl;
     const double am = -0.5 * w * gxm, cp = -0.5 * w * gxp;
-    const double inv = 1.0 / (M - am - cp);
-    a[q] = valid ? am * inv : 0.0;
-    c[q] = valid ? cp * inv : 0.0;
-    r[q] = valid ? rhs * inv : 0.0;
+    seg.row(q, valid ? am : 0.0, valid ? M - am - cp : 1.0, valid ? cp : 0.0, valid ? rhs : 0.0);
   }
   double x[CPL];
-  group_solve<CPL, G>(a, c, r, x, lig);
+  group_solve<CPL, G>(seg, x, lig);
   if (lvalid) {
 #pragma unroll
     for (int q = 0; q < CPL; q++)
       if (i0 + q < nx) Ts[base + i0 + q] = x[q];
+  }
+}
+
+// x sweep, two-phase form.  Phase 1 forms each cell's right-hand side with
+// lanes running along x (coalesced loads of T^n and its four off-axis
+// neighbour rows, U cells per thread per batch, all loads of a batch issued
+// before any is used) and parks it in LDS; phase 2 reads it back as G lanes x
+// SEG consecutive cells per line, recomputes the x-axis coefficients from
+// the class bytes and solves like k_sweep_x.  LPB lines per block.  SRC adds
+// the volumetric source terms (only on the rare steps that carry one).
+template <class V, int SEG, int G, int LPB, bool SRC>
+__global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
+                                                    const double *__restrict__ Tn,
+                                                    double *__restrict__ Ts,
+                                                    const uint8_t *__restrict__ vmap,
+                                                    const double *__restrict__ vtab_g,
+                                                    const double *__restrict__ sfield) {
+  constexpr int NT = G * LPB;
+  constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;  // padded LDS row (doubles)
+  constexpr int U = 4;                                // cells per thread per batch
+  extern __shared__ double lds[];
+  __shared__ int lbase[LPB][5];    // own, y-1, y+1, z-1, z+1 row bases
+  __shared__ double lmask[LPB][4];  // 1/0: neighbour exists at y-1, y+1, z-1, z+1
+  view.load_tables(lds, -1);
+  const int tabpad = (view.lds_doubles(-1) + 1) & ~1;
+  double *vtab = lds + tabpad;
+  double *rbuf = vtab + (SRC ? kMaxVtab : 0);
+  if (SRC && vmap)
+    for (int t = threadIdx.x; t < kMaxVtab; t += NT) vtab[t] = vtab_g[t];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int nlines = nz * ny;
+  const int line0 = xcd_remap(blockIdx.x, gridDim.x) * LPB;
+  const int tid = threadIdx.x;
+  if (tid < LPB) {
+    const int line = line0 + tid;
+    const int lc = line < nlines ? line : nlines - 1;
+    const int k = lc / ny, j = lc - k * ny;
+    lbase[tid][0] = lc * nx;
+    lbase[tid][1] = (j > 0 ? lc - 1 : lc) * nx;
+    lbase[tid][2] = (j < ny - 1 ? lc + 1 : lc) * nx;
+    lbase[tid][3] = (k > 0 ? lc - ny : lc) * nx;
+    lbase[tid][4] = (k < nz - 1 ? lc + ny : lc) * nx;
+    lmask[tid][0] = j > 0 ? 1.0 : 0.0;
+    lmask[tid][1] = j < ny - 1 ? 1.0 : 0.0;
+    lmask[tid][2] = k > 0 ? 1.0 : 0.0;
+    lmask[tid][3] = k < nz - 1 ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const double *tabz = view.table(lds, 0), *taby = view.table(lds, 1), *tabx = view.table(lds, 2);
+
+  // ---- phase 1: right-hand sides, lanes along x
+  if ((nx & 3) == 0) {
+    // quads: 4 consecutive cells per thread, vector loads; quad e = tid + NT*u
+    // of the block's LPB x nx/4 quads, walked incrementally as (l, q)
+    constexpr int UQ = 2;
+    const int qpl = nx >> 2;
+    const int total = LPB * qpl;
+    int l = tid / qpl, qx = tid - (tid / qpl) * qpl;
+    const int dl = NT / qpl, dq = NT - (NT / qpl) * qpl;
+    for (int e0 = 0; e0 < total; e0 += UQ * NT) {
+      int li[UQ], xq[UQ];
+#pragma unroll
+      for (int b = 0; b < UQ; b++) {
+        li[b] = l;
+        xq[b] = qx;
+        qx += dq;
+        l += dl;
+        if (qx >= qpl) { qx -= qpl; l++; }
+      }
+      double T[UQ][4], Tym[UQ][4], Typ[UQ][4], Tzm[UQ][4], Tzp[UQ][4], TL[UQ], TR[UQ];
+      typename V::Raw rx[UQ][4], rxn[UQ], ry[UQ][4], ryp[UQ][4], rz[UQ][4], rzp[UQ][4];
+#pragma unroll
+      for (int b = 0; b < UQ; b++) {
+        const int lb = li[b] < LPB ? li[b] : LPB - 1;
+        const int x0 = xq[b] << 2;
+        const int b0 = lbase[lb][0];
+        const double2 *p0 = reinterpret_cast<const double2 *>(Tn + b0 + x0);
+        const double2 *p1 = reinterpret_cast<const double2 *>(Tn + lbase[lb][1] + x0);
+        const double2 *p2 = reinterpret_cast<const double2 *>(Tn + lbase[lb][2] + x0);
+        const double2 *p3 = reinterpret_cast<const double2 *>(Tn + lbase[lb][3] + x0);
+        const double2 *p4 = reinterpret_cast<const double2 *>(Tn + lbase[lb][4] + x0);
+        double2 v;
+        v = p0[0]; T[b][0] = v.x; T[b][1] = v.y; v = p0[1]; T[b][2] = v.x; T[b][3] = v.y;
+        v = p1[0]; Tym[b][0] = v.x; Tym[b][1] = v.y; v = p1[1]; Tym[b][2] = v.x; Tym[b][3] = v.y;
+        v = p2[0]; Typ[b][0] = v.x; Typ[b][1] = v.y; v = p2[1]; Typ[b][2] = v.x; Typ[b][3] = v.y;
+        v = p3[0]; Tzm[b][0] = v.x; Tzm[b][1] = v.y; v = p3[1]; Tzm[b][2] = v.x; Tzm[b][3] = v.y;
+        v = p4[0]; Tzp[b][0] = v.x; Tzp[b][1] = v.y; v = p4[1]; Tzp[b][2] = v.x; Tzp[b][3] = v.y;
+        TL[b] = Tn[b0 + (x0 > 0 ? x0 - 1 : 0)];
+        TR[b] = Tn[b0 + (x0 + 4 < nx ? x0 + 4 : nx - 1)];
+        view.fetch4(2, b0 + x0, rx[b]);
+        rxn[b] = view.fetch(2, b0 + (x0 + 4 < nx ? x0 + 4 : nx - 1));
+        view.fetch4(1, b0 + x0, ry[b]);
+        view.fetch4(1, lbase[lb][2] + x0, ryp[b]);
+        view.fetch4(0, b0 + x0, rz[b]);
+        view.fetch4(0, lbase[lb][4] + x0, rzp[b]);
+      }
+#pragma unroll
+      for (int b = 0; b < UQ; b++) {
+        const int lb = li[b] < LPB ? li[b] : LPB - 1;
+        const int x0 = xq[b] << 2;
+        Entry ex[5];
+#pragma unroll
+        for (int q = 0; q < 4; q++) ex[q] = view.resolve(tabx, rx[b][q]);
+        ex[4] = view.resolve(tabx, rxn[b]);
+        double out[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int x = x0 + q;
+          const double T0 = T[b][q];
+          const double tl = q > 0 ? T[b][q - 1] : TL[b];
+          const double tr = q < 3 ? T[b][q + 1] : TR[b];
+          const double gxm = (x > 0 ? 1.0 : 0.0) * ex[q].g;
+          const double gxp = (x < nx - 1 ? 1.0 : 0.0) * ex[q + 1].g;
+          const double gym = lmask[lb][0] * view.resolve(taby, ry[b][q]).g;
+          const double gyp = lmask[lb][1] * view.resolve(taby, ryp[b][q]).g;
+          const double gzm = lmask[lb][2] * view.resolve(tabz, rz[b][q]).g;
+          const double gzp = lmask[lb][3] * view.resolve(tabz, rzp[b][q]).g;
+          double expl = 0.5 * (gxm * (tl - T0) + gxp * (tr - T0)) + gym * (Tym[b][q] - T0) +
+                        gyp * (Typ[b][q] - T0) + gzm * (Tzm[b][q] - T0) + gzp * (Tzp[b][q] - T0);
+          if (SRC) {
+            if (vmap) expl += vtab[vmap[lbase[lb][0] + x]];
+            if (sfield) expl += sfield[lbase[lb][0] + x];
+          }
+          out[q] = ex[q].M * T0 + ex[q].w * expl;
+        }
+        if (li[b] < LPB) {
+          double *dst = rbuf + li[b] * RS + x0 + (x0 >> 4);
+#pragma unroll
+          for (int q = 0; q < 4; q++) dst[q] = out[q];
+        }
+      }
+    }
+  } else {
+    // general nx: one cell per thread per item, U items per batch
+    constexpr int U = 4;
+    const int total = LPB * nx;
+    int l = tid / nx, i = tid - (tid / nx) * nx;
+    const int dl = NT / nx, di = NT - (NT / nx) * nx;
+    for (int e0 = 0; e0 < total; e0 += U * NT) {
+      int li[U], ii[U];
+#pragma unroll
+      for (int b = 0; b < U; b++) {
+        li[b] = l;
+        ii[b] = i;
+        i += di;
+        l += dl;
+        if (i >= nx) { i -= nx; l++; }
+      }
+      double Tc[U], Txm[U], Txp[U], Tym[U], Typ[U], Tzm[U], Tzp[U];
+      typename V::Raw rx[U], rxp[U], ry[U], ryp[U], rz[U], rzp[U];
+#pragma unroll
+      for (int b = 0; b < U; b++) {
+        const int lb = li[b] < LPB ? li[b] : LPB - 1;
+        const int x = ii[b];
+        const int xm = x > 0 ? x - 1 : x, xp = x < nx - 1 ? x + 1 : x;
+        const int b0 = lbase[lb][0];
+        Tc[b] = Tn[b0 + x];
+        Txm[b] = Tn[b0 + xm];
+        Txp[b] = Tn[b0 + xp];
+        Tym[b] = Tn[lbase[lb][1] + x];
+        Typ[b] = Tn[lbase[lb][2] + x];
+        Tzm[b] = Tn[lbase[lb][3] + x];
+        Tzp[b] = Tn[lbase[lb][4] + x];
+        rx[b] = view.fetch(2, b0 + x);
+        rxp[b] = view.fetch(2, b0 + xp);
+        ry[b] = view.fetch(1, b0 + x);
+        ryp[b] = view.fetch(1, lbase[lb][2] + x);
+        rz[b] = view.fetch(0, b0 + x);
+        rzp[b] = view.fetch(0, lbase[lb][4] + x);
+      }
+#pragma unroll
+      for (int b = 0; b < U; b++) {
+        const int lb = li[b] < LPB ? li[b] : LPB - 1;
+        const int x = ii[b];
+        const double mxm = x > 0 ? 1.0 : 0.0, mxp = x < nx - 1 ? 1.0 : 0.0;
+        const Entry e = view.resolve(tabx, rx[b]);
+        const double gxm = mxm * e.g;
+        const double gxp = mxp * view.resolve(tabx, rxp[b]).g;
+        const double gym = lmask[lb][0] * view.resolve(taby, ry[b]).g;
+        const double gyp = lmask[lb][1] * view.resolve(taby, ryp[b]).g;
+        const double gzm = lmask[lb][2] * view.resolve(tabz, rz[b]).g;
+        const double gzp = lmask[lb][3] * view.resolve(tabz, rzp[b]).g;
+        const double T0 = Tc[b];
+        double expl = 0.5 * (gxm * (Txm[b] - T0) + gxp * (Txp[b] - T0)) + gym * (Tym[b] - T0) +
+                      gyp * (Typ[b] - T0) + gzm * (Tzm[b] - T0) + gzp * (Tzp[b] - T0);
+        if (SRC) {
+          if (vmap) expl += vtab[vmap[lbase[lb][0] + x]];
+          if (sfield) expl += sfield[lbase[lb][0] + x];
+        }
+        if (li[b] < LPB) rbuf[li[b] * RS + x + (x >> 4)] = e.M * T0 + e.w * expl;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: G lanes x SEG cells per line
+  const int lq = tid / G, lig = tid % G;
+  const bool lvalid = line0 + lq < nlines;
+  const int base = lbase[lq][0];
+  const int i0 = lig * SEG;
+  typename V::Raw rxs[SEG];
+  double rh[SEG];
+  if constexpr (SEG % 4 == 0) {
+    if ((nx & 3) == 0 && i0 + SEG <= nx) {
+#pragma unroll
+      for (int q = 0; q < SEG; q += 4) {
+        typename V::Raw t4[4];
+        view.fetch4(2, base + i0 + q, t4);
+#pragma unroll
+        for (int u = 0; u < 4; u++) rxs[q + u] = t4[u];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < SEG; q++) rxs[q] = view.fetch(2, base + min(i0 + q, nx - 1));
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < SEG; q++) rxs[q] = view.fetch(2, base + min(i0 + q, nx - 1));
+  }
+#pragma unroll
+  for (int q = 0; q < SEG; q++) {
+    const int x = min(i0 + q, nx - 1);
+    rh[q] = rbuf[lq * RS + x + (x >> 4)];
+  }
+  Entry ex[SEG + 1];
+#pragma unroll
+  for (int q = 0; q < SEG; q++) ex[q] = view.resolve(tabx, rxs[q]);
+  ex[SEG].g = LaneX<G>::down(ex[0].g, 1, lig);
+  SegOf<SEG> seg;
+#pragma unroll
+  for (int q = 0; q < SEG; q++) {
+    const int x = i0 + q;
+    const bool valid = lvalid && x < nx;
+    const double gxm = x > 0 ? ex[q].g : 0.0;
+    const double gxp = x < nx - 1 ? ex[q + 1].g : 0.0;
+    const double am = -0.5 * ex[q].w * gxm, cp = -0.5 * ex[q].w * gxp;
+    seg.row(q, valid ? am : 0.0, valid ? ex[q].M - am - cp : 1.0, valid ? cp : 0.0,
+            valid ? rh[q] : 0.0);
+  }
+  double xs[SEG];
+  group_solve<SEG, G>(seg, xs, lig);
+  // ---- write back through LDS so that the global stores are coalesced
+  // (each lane owns SEG consecutive cells; its reads of rbuf above touched
+  // only those cells)
+#pragma unroll
+  for (int q = 0; q < SEG; q++) {
+    const int x = i0 + q;
+    if (x < nx) rbuf[lq * RS + x + (x >> 4)] = xs[q];
+  }
+  __syncthreads();
+  if ((nx & 3) == 0) {
+    const int qpl = nx >> 2;
+    for (int e = tid; e < LPB * qpl; e += NT) {
+      const int l2 = e / qpl, x0 = (e - l2 * qpl) << 2;
+      if (line0 + l2 < nlines) {
+        const double *src = rbuf + l2 * RS + x0 + (x0 >> 4);
+        double2 *dst = reinterpret_cast<double2 *>(Ts + lbase[l2][0] + x0);
+        dst[0] = make_double2(src[0], src[1]);
+        dst[1] = make_double2(src[2], src[3]);
+      }
+    }
+  } else {
+    for (int e = tid; e < LPB * nx; e += NT) {
+      const int l2 = e / nx, x = e - l2 * nx;
+      if (line0 + l2 < nlines) Ts[lbase[l2][0] + x] = rbuf[l2 * RS + x + (x >> 4)];
+    }
   }
 }
 
@@ -167,7 +430,8 @@ __global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int axis,
 #pragma unroll
   for (int q = 0; q <= SEG; q++) raw[q] = view.fetch(axis, base + min(t0 + q, L - 1) * stride);
 
-  // ---- rows
+  // ---- rows, normalised to unit diagonal (independent of each other, so the
+  // divisions overlap), then reduced to the segment's two end rows
   double a[SEG], c[SEG], r[SEG];
   Entry en = view.resolve(lds, raw[0]);
 #pragma unroll
@@ -289,17 +553,35 @@ static hipError_t launch_x_t(const Grid &g, const V &v, const double *Tn, double
   return hipGetLastError();
 }
 
+template <class V, int SEG, int G, int LPB>
+static hipError_t launch_x2_t(const Grid &g, const V &v, const double *Tn, double *Ts,
+                              const uint8_t *vmap, const double *vtab, const double *sfield,
+                              hipStream_t s) {
+  const int nlines = g.nz * g.ny;
+  const int blocks = (nlines + LPB - 1) / LPB;
+  constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;
+  const bool src = vmap || sfield;
+  const size_t shm =
+      sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + (src ? kMaxVtab : 0) + LPB * RS);
+  if (src)
+    hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, true>), dim3(blocks), dim3(G * LPB), shm, s, g,
+                       v, Tn, Ts, vmap, vtab, sfield);
+  else
+    hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false>), dim3(blocks), dim3(G * LPB), shm, s, g,
+                       v, Tn, Ts, vmap, vtab, sfield);
+  return hipGetLastError();
+}
+
 XConfig x_config_for(int nx) {
-  // G lanes x CPL cells per line; defaults keep a lane at <= 4 cells (about
-  // 128 VGPRs); the autotuner may pick others
-  if (nx <= 16) return {1, 16};
-  if (nx <= 32) return {2, 16};
-  if (nx <= 64) return {4, 16};
-  if (nx <= 128) return {4, 32};
-  if (nx <= 256) return {4, 64};
-  if (nx <= 512) return {8, 64};
-  if (nx <= 1024) return {16, 64};
-  return {0, 0};
+  // two-phase kernel, 16 lanes per line where a line fits 16 x 16 cells
+  if (nx <= 16) return {1, 16, 8};
+  if (nx <= 32) return {2, 16, 8};
+  if (nx <= 64) return {4, 16, 8};
+  if (nx <= 128) return {8, 16, 8};
+  if (nx <= 256) return {16, 16, 8};
+  if (nx <= 512) return {16, 32, 4};
+  if (nx <= 1024) return {16, 64, 2};
+  return {0, 0, 0};
 }
 
 template <class V>
@@ -307,12 +589,19 @@ static hipError_t launch_x_v(XConfig cfg, const Grid &g, const V &v, const doubl
                              double *Ts, const uint8_t *vmap, const double *vtab,
                              const double *sfield, hipStream_t s) {
 #define HS2_X(C, GG)                                                               \
-  if (cfg.cpl == C && cfg.g == GG)                                                 \
+  if (cfg.lpb == 0 && cfg.cpl == C && cfg.g == GG)                                 \
     return launch_x_t<V, C, GG>(g, v, Tn, Ts, vmap, vtab, sfield, s);
   HS2_X(1, 16) HS2_X(2, 16) HS2_X(4, 16) HS2_X(8, 16) HS2_X(16, 16)
   HS2_X(2, 32) HS2_X(4, 32) HS2_X(8, 32) HS2_X(16, 32)
   HS2_X(2, 64) HS2_X(4, 64) HS2_X(8, 64) HS2_X(16, 64)
 #undef HS2_X
+#define HS2_X2(C, GG, LL)                                                          \
+  if (cfg.lpb == LL && cfg.cpl == C && cfg.g == GG)                                \
+    return launch_x2_t<V, C, GG, LL>(g, v, Tn, Ts, vmap, vtab, sfield, s);
+  HS2_X2(1, 16, 8) HS2_X2(2, 16, 8) HS2_X2(4, 16, 8) HS2_X2(8, 16, 8) HS2_X2(16, 16, 8)
+  HS2_X2(4, 16, 16) HS2_X2(8, 16, 16) HS2_X2(16, 16, 4) HS2_X2(16, 16, 16)
+  HS2_X2(8, 32, 4) HS2_X2(16, 32, 4) HS2_X2(16, 32, 2) HS2_X2(16, 64, 2) HS2_X2(8, 64, 2)
+#undef HS2_X2
   return hipErrorInvalidValue;
 }
 

@@ -6,8 +6,8 @@
 
 namespace hs2 {
 
-struct XConfig {  // x sweep: CPL cells per lane, G lanes per line
-  int cpl, g;
+struct XConfig {  // x sweep: CPL cells per lane, G lanes per line, LPB lines per
+  int cpl, g, lpb;  // block of the two-phase kernel (0: single-phase kernel)
 };
 struct SConfig {  // strided sweep: SEG cells per thread, CW lines per block
   int seg, cw;

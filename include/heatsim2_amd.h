@@ -5,9 +5,9 @@
  *   - every function returns an int status (HS2_OK = 0) and never exits or
  *     aborts; hs2_last_error() gives a message for the calling thread;
  *   - every large buffer is caller-owned device memory passed as a plain
- *     pointer (fields are (nz,ny,nx) C-order float64; maps are uint8 in the
- *     upstream layout); the plan owns only its small tables and one n-double
- *     workspace;
+ *     pointer (fields are (nz,ny,nx) C-order float64, 16-byte aligned; maps
+ *     are uint8 in the upstream layout); the plan owns its tables, the
+ *     per-cell coefficient classes and one n-double workspace;
  *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
  *   - a plan is used from one stream at a time; distinct plans are
  *     independent.
@@ -89,16 +89,18 @@ int hs2_plan_set_dense(hs2_plan *plan, const double *d_M, const uint8_t *d_w, co
 int hs2_plan_expand(hs2_plan *plan, double *d_M, double *d_w, double *d_gz, double *d_gy,
                     double *d_gx, void *stream);
 
-/* Kernel launch shapes (used by the autotuner): the x sweep runs x_lanes
- * lanes of x_cpl consecutive cells per x-line (x_cpl*x_lanes >= nx;
- * x_lanes in {16,32,64}, x_cpl in {1,2,4,8,16}); the y and z sweeps split
- * each line into segments of *_seg cells (one thread each) and put *_lines
- * lines in a block (segments*lines <= 512).  No upstream counterpart. */
-int hs2_plan_set_config(hs2_plan *plan, int x_cpl, int x_lanes, int y_seg, int y_lines,
-                        int z_seg, int z_lines);
+/* Kernel launch shapes (used by the autotuner).  x sweep: x_lanes lanes of
+ * x_cpl consecutive cells per x-line (x_cpl*x_lanes >= nx; x_lanes in
+ * {16,32,64}, x_cpl in {1,2,4,8,16}); x_lines > 0 selects the two-phase
+ * kernel with x_lines lines per block (right-hand sides staged in LDS),
+ * x_lines = 0 the single-phase one.  y and z sweeps: segments of *_seg cells
+ * per thread, *_lines lines per block (segments*lines <= 512).  No upstream
+ * counterpart. */
+int hs2_plan_set_config(hs2_plan *plan, int x_cpl, int x_lanes, int x_lines, int y_seg,
+                        int y_lines, int z_seg, int z_lines);
 
-/* info[0..9] = mode (0 none, 1 class maps, 2 dense), classes per axis z/y/x,
- * x_cpl, x_lanes, y_seg, y_lines, z_seg, z_lines. */
+/* info[0..10] = mode (0 none, 1 class maps, 2 dense), classes per axis z/y/x,
+ * x_cpl, x_lanes, y_seg, y_lines, z_seg, z_lines, x_lines. */
 int hs2_plan_info(const hs2_plan *plan, int *info);
 
 /* One full Douglas step T^n -> T^{n+1} (x, y, z substeps).

@@ -216,7 +216,8 @@ def test_every_launch_shape_matches_oracle(hs, shape):
     ref = oracle.run_ref(case)
     rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
     info = _lib.plan_info(P.plan)
-    base = {"x": (info["x_cpl"], info["x_lanes"]), "y": (info["y_seg"], info["y_lines"]),
+    base = {"x": (info["x_cpl"], info["x_lanes"], info["x_lines"]),
+            "y": (info["y_seg"], info["y_lines"]),
             "z": (info["z_seg"], info["z_lines"])}
     nz, ny, nx = shape
     trials = [("x", c) for c in autotune.x_candidates(nx)] + \
