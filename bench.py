@@ -14,9 +14,13 @@ maps resident in HBM, bracketed by torch.cuda.synchronize().
 
 roofline: the dominant sweep kernel's algorithmic bytes per launch (x: 16 B
 per cell, y and z: 24 B per cell -- the FP64 field passes of the delta-form
-step, 64 B per cell-step in all) over its mean launch time from HIP events
-recorded on the launch stream around every substep kernel of K further steps
-(hs2_run_timed); peak = 8 TB/s.
+step, 64 B per cell-step in all) over its mean launch duration, from HIP
+events recorded on the launch stream around every substep kernel of K more
+steps run right after the timed region (hs2_run_timed; each event pair adds a
+few microseconds of its own, so the achieved rate is slightly understated);
+peak = 8 TB/s.  traffic: the HBM bytes per launch of that kernel read from
+the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+(profiles/hbm_traffic_two_layer_256.txt, copy-calibrated).
 
 cpu_baseline: the oracle's reference-structured step (assembled permuted
 CSR operators, global Thomas with precomputed LU, transposes -- the
@@ -37,6 +41,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = (16.0, 24.0, 24.0)  # x, y, z sweep algorithmic bytes
+TRAFFIC_SUMMARY = "hbm_traffic_two_layer_256.txt"
 METRIC = "FP64 ADI cell-steps/s @256³ + batched fwd+adjoint gradients/s vs plain"
 
 
@@ -70,7 +75,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, nargs=3, default=(256, 256, 256))
     ap.add_argument("--no-tune", action="store_true")
@@ -149,9 +154,17 @@ def main():
         log("  %s: %.4f ms/launch, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
             % (names[i], kms[i], gbs[i], 100 * gbs[i] / HBM_PEAK_GBS))
     step_gbs = 64.0 * value / 1e9
+    # measured HBM bytes per launch: committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # summary of this workload (copy-calibrated), when it is for this size
+    from heatsim2_amd.rocprof_summary import read_traffic
+    tr = read_traffic(os.path.join(ROOT, "profiles", TRAFFIC_SUMMARY))
+    traffic = None
+    if tr and tr.get("cells") == n:
+        traffic = tr["traffic_bytes"].get(names[dom])
     roofline = dict(bound="hbm", kernel=names[dom], achieved=round(gbs[dom], 1),
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs[dom] / HBM_PEAK_GBS, 4),
-                    traffic=None, bytes_per_launch=BYTES_PER_CELL[dom] * n,
+                    traffic=traffic, traffic_source=TRAFFIC_SUMMARY if traffic else None,
+                    bytes_per_launch=BYTES_PER_CELL[dom] * n,
                     kernel_ms=[round(x, 5) for x in kms],
                     step_achieved=round(step_gbs, 1), step_frac=round(step_gbs / HBM_PEAK_GBS, 4))
 

@@ -394,10 +394,11 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
 // line stride nx*ny.  Solves (M - La/2) X = M Y - (La/2) T^n along each line.
 // X may alias Y or T^n: a block owns whole lines, and all of its loads
 // happen before its first barrier and all of its stores after its last.
-template <class V, int SEG>
-__global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int axis, int CW,
-                                                       const double *Tn, const double *Y,
-                                                       double *X, double *surf, double dz) {
+template <class V, int SEG, int AXIS>
+__global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, const double *Tn,
+                                                       const double *Y, double *X, double *surf,
+                                                       double dz) {
+  constexpr int axis = AXIS;
   extern __shared__ double lds[];
   view.load_tables(lds, axis);
   const int P = blockDim.x / CW;
@@ -634,8 +635,13 @@ static hipError_t launch_s_t(const Grid &g, const V &v, int axis, int CW, const 
   const int P = (L + SEG - 1) / SEG;
   const int other = axis == 1 ? g.nz : g.ny;
   dim3 grid((g.nx + CW - 1) / CW, other);
-  hipLaunchKernelGGL((k_sweep_strided<V, SEG>), grid, dim3(P * CW), strided_lds_bytes_t<V>(v, axis, P, CW), s,
-                     g, v, axis, CW, Tn, Y, X, surf, dz);
+  const size_t lds = strided_lds_bytes_t<V>(v, axis, P, CW);
+  if (axis == 1)
+    hipLaunchKernelGGL((k_sweep_strided<V, SEG, 1>), grid, dim3(P * CW), lds, s, g, v, CW, Tn, Y,
+                       X, surf, dz);
+  else
+    hipLaunchKernelGGL((k_sweep_strided<V, SEG, 0>), grid, dim3(P * CW), lds, s, g, v, CW, Tn, Y,
+                       X, surf, dz);
   return hipGetLastError();
 }
 

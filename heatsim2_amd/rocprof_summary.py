@@ -1,0 +1,181 @@
+"""Summaries of rocprofv3 CSV output (``-f csv``) for the ADI sweep kernels.
+
+    python -m heatsim2_amd.rocprof_summary trace DIR
+        per-kernel launch statistics (the *_kernel_stats.csv of --stats) and
+        mean device time per sweep (x, y, z) from *_kernel_trace.csv
+    python -m heatsim2_amd.rocprof_summary traffic --fetch DIR --write DIR \
+            --copy-fetch DIR --copy-write DIR --copy-bytes B --cells N
+        HBM bytes per sweep launch from FETCH_SIZE / WRITE_SIZE (KiB per
+        dispatch), each scaled by what the same counter reads for a plain
+        device copy of B bytes
+
+Kernels are attributed to sweeps by name (the strided kernel's last
+template argument is its axis).  The last line of the traffic
+summary is ``json: {...}`` with the per-sweep traffic in bytes; bench.py
+reads it.
+"""
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import re
+import sys
+
+BYTES_PER_CELL = {"x": 16.0, "y": 24.0, "z": 24.0}
+
+
+def _one(directory, suffix):
+    hits = sorted(glob.glob(os.path.join(directory, "**", "*" + suffix), recursive=True))
+    if not hits:
+        raise SystemExit("no *%s under %s" % (suffix, directory))
+    return hits[0]
+
+
+def _rows(path):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def short_name(name):
+    name = name.replace("(anonymous namespace)::", "")
+    name = re.sub(r"\(.*", "", name).replace("void ", "").replace("hs2::", "")
+    return name.strip()
+
+
+_STRIDED = re.compile(r"k_sweep_strided<[^<>]*,\s*(\d+)>")
+
+
+def sweep_of(name):
+    """'x', 'y' or 'z' for a sweep kernel's name, else None.  The strided
+    kernel carries its axis (1: y, 0: z) as its last template argument."""
+    if "k_sweep_x" in name:
+        return "x"
+    m = _STRIDED.search(name)
+    if m:
+        return "y" if m.group(1) == "1" else "z"
+    return None
+
+
+def sweep_of_dispatches(dispatches):
+    """dispatches: [(dispatch_id, kernel_name, value)] -> {"x"|"y"|"z": [values]}.
+
+    Per sweep only the most frequently launched instantiation counts (the
+    source-free step kernel; the flash step's source variant is excluded)."""
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for _, name, value in sorted(dispatches):
+        s = sweep_of(name)
+        if s:
+            per[s][name].append(value)
+    return {s: max(by.values(), key=len) for s, by in per.items()}
+
+
+def trace_summary(directory):
+    lines = []
+    stats = _rows(_one(directory, "_kernel_stats.csv"))
+    lines.append("%-58s %7s %11s %11s %11s" % ("kernel", "calls", "avg_us", "min_us", "max_us"))
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:12]:
+        lines.append("%-58s %7d %11.2f %11.2f %11.2f" % (
+            short_name(r["Name"])[:58], int(r["Calls"]), float(r["AverageNs"]) / 1e3,
+            float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3))
+    trace = _rows(_one(directory, "_kernel_trace.csv"))
+    disp = [(int(r["Dispatch_Id"]), r["Kernel_Name"],
+             (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3) for r in trace]
+    per = sweep_of_dispatches(disp)
+    lines.append("")
+    lines.append("per sweep: mean device time of the most launched instantiation")
+    for s in ("x", "y", "z"):
+        v = per.get(s, [])
+        if v:
+            lines.append("  sweep_%s: %d launches, mean %.2f us, min %.2f us" % (
+                s, len(v), sum(v) / len(v), min(v)))
+    return "\n".join(lines), {s: (sum(v) / len(v) if v else None) for s, v in per.items()}
+
+
+def _counter(directory, counter):
+    rows = _rows(_one(directory, "_counter_collection.csv"))
+    disp = collections.defaultdict(float)
+    names = {}
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        d = int(r["Dispatch_Id"])
+        disp[d] += float(r["Counter_Value"])
+        names[d] = r["Kernel_Name"]
+    return [(d, names[d], v) for d, v in disp.items()]
+
+
+def _copy_reading(directory, counter):
+    """Mean reading of the most frequently launched kernel (the copy)."""
+    disp = _counter(directory, counter)
+    by = collections.defaultdict(list)
+    for _, name, v in disp:
+        by[name].append(v)
+    name, vals = max(by.items(), key=lambda kv: len(kv[1]))
+    return short_name(name), sum(vals) / len(vals), len(vals)
+
+
+def traffic_summary(a):
+    lines = []
+    cname, cf, nf = _copy_reading(a.copy_fetch, "FETCH_SIZE")
+    _, cw, nw = _copy_reading(a.copy_write, "WRITE_SIZE")
+    ff = a.copy_bytes / (cf * 1024.0)
+    fw = a.copy_bytes / (cw * 1024.0)
+    lines.append("calibration: %s over %d B: FETCH_SIZE %.0f KiB (n=%d), WRITE_SIZE %.0f KiB (n=%d)"
+                 % (cname, a.copy_bytes, cf, nf, cw, nw))
+    lines.append("scale factors: fetch x%.3f, write x%.3f" % (ff, fw))
+    fetch = sweep_of_dispatches(_counter(a.fetch, "FETCH_SIZE"))
+    write = sweep_of_dispatches(_counter(a.write, "WRITE_SIZE"))
+    lines.append("")
+    lines.append("%-8s %12s %12s %14s %14s %14s %10s" % (
+        "sweep", "FETCH_KiB", "WRITE_KiB", "read_MB", "write_MB", "traffic_MB", "algo_MB"))
+    out = {}
+    for s in ("x", "y", "z"):
+        if not fetch.get(s) or not write.get(s):
+            continue
+        f = sum(fetch[s]) / len(fetch[s])
+        w = sum(write[s]) / len(write[s])
+        rb, wb = f * 1024.0 * ff, w * 1024.0 * fw
+        algo = BYTES_PER_CELL[s] * a.cells
+        out["sweep_" + s] = rb + wb
+        lines.append("%-8s %12.0f %12.0f %14.1f %14.1f %14.1f %10.1f" % (
+            "sweep_" + s, f, w, rb / 1e6, wb / 1e6, (rb + wb) / 1e6, algo / 1e6))
+    lines.append("")
+    lines.append("json: " + json.dumps({"cells": a.cells, "fetch_scale": round(ff, 4),
+                                        "write_scale": round(fw, 4),
+                                        "traffic_bytes": {k: round(v) for k, v in out.items()}}))
+    return "\n".join(lines)
+
+
+def read_traffic(path):
+    """The json line of a committed traffic summary, or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("json: "):
+                    return json.loads(line[6:])
+    except OSError:
+        return None
+    return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("dir")
+    r = sub.add_parser("traffic")
+    for k in ("fetch", "write", "copy_fetch", "copy_write"):
+        r.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
+    r.add_argument("--copy-bytes", type=int, required=True)
+    r.add_argument("--cells", type=int, required=True)
+    a = ap.parse_args(argv)
+    if a.cmd == "trace":
+        print(trace_summary(a.dir)[0])
+    else:
+        print(traffic_summary(a))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
