@@ -22,6 +22,7 @@ import heatsim2_amd.hs2_indexing as hs2_indexing
 import heatsim2_amd.crank_nicolson as crank_nicolson
 import heatsim2_amd.alternatingdirection as alternatingdirection
 import heatsim2_amd.autotune as autotune
+import heatsim2_amd.adjoint as adjoint
 
 __version__ = "0.1.0"
 

@@ -32,7 +32,8 @@ class HS2Error(RuntimeError):
 # every symbol include/heatsim2_amd.h declares
 EXPORTS = ("hs2_plan_create", "hs2_plan_destroy", "hs2_plan_set_compact", "hs2_plan_set_dense",
            "hs2_plan_expand", "hs2_plan_set_config", "hs2_plan_info", "hs2_step", "hs2_run",
-           "hs2_run_timed", "hs2_surface_temperature", "hs2_last_error", "hs2_version")
+           "hs2_run_timed", "hs2_grad_work_doubles", "hs2_grad_forward", "hs2_grad_backward",
+           "hs2_surface_temperature", "hs2_last_error", "hs2_version")
 
 _lib = None
 
@@ -67,6 +68,13 @@ def load():
     L.hs2_run.restype = i32
     L.hs2_run_timed.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_double), vp]
     L.hs2_run_timed.restype = i32
+    L.hs2_grad_work_doubles.argtypes = [vp, i32]
+    L.hs2_grad_work_doubles.restype = ctypes.c_size_t
+    L.hs2_grad_forward.argtypes = [vp, vp, i32, i32, vp, vp, vp, i32, vp, vp]
+    L.hs2_grad_forward.restype = i32
+    L.hs2_grad_backward.argtypes = [vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                    ctypes.c_size_t, vp]
+    L.hs2_grad_backward.restype = i32
     L.hs2_surface_temperature.argtypes = [vp, i32, i32, i32, dbl, vp, vp]
     L.hs2_surface_temperature.restype = i32
     L.hs2_last_error.argtypes = []

@@ -1,0 +1,313 @@
+// Adjoint of the Douglas ADI step (gfx950, FP64).
+//
+// Forward step, per cell (M = rho*c/dt, w = 1 COMPUTE / 0 FIXED, L_a the
+// weighted face Laplacian of axis a, L_a = W G_a with G_a symmetric):
+//   x: A_x T*      = (M + L_x/2 + L_y + L_z) T^n + w S      A_a = M - L_a/2
+//   y: A_y T**     = M T*  - L_y/2 T^n
+//   z: A_z T^{n+1} = M T** - L_z/2 T^n
+// Given lam = dJ/dT^{n+1}, the adjoint runs the transposed solves in reverse
+// axis order,
+//   A_z^T u_z = lam,   A_y^T u_y = M u_z,   A_x^T u_x = M u_y,
+// and then the adjoint stencil
+//   dJ/dT^n = M u_x + G_x(w u_x / 2) + G_y(w (u_x - u_y / 2)) + G_z(w (u_x - u_z / 2))
+// with the parameter sensitivities of this step (D(v) = v_c - v_m across the
+// minus face of c on that axis, E(u) = w_m u_m - w_c u_c):
+//   dJ/dM_c   -= u_x (T* - T^n) + u_y (T** - T*) + u_z (T^{n+1} - T**)
+//   dJ/dg_x   += D(T* + T^n) E(u_x) / 2
+//   dJ/dg_y   += D(T^n) E(u_x) + D(T** - T^n) E(u_y) / 2
+//   dJ/dg_z   += D(T^n) E(u_x) + D(T^{n+1} - T^n) E(u_z) / 2
+//   dJ/dS_c    = w u_x
+// Row t of A_a^T along a line: diagonal M_t + w_t (g_t + g_{t+1}) / 2,
+// sub-diagonal -w_{t-1} g_t / 2, super-diagonal -w_{t+1} g_{t+1} / 2 (g_t the
+// conductance of the face between t-1 and t).  A_a is strictly row
+// diagonally dominant, so A_a^T is strictly column dominant and the same
+// segment elimination + PCR solves it without pivoting.
+//
+// No upstream counterpart: heatsim2 has no gradients.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "hs2_adjoint.h"
+
+namespace hs2 {
+
+// ------------------------------------------------- transposed line solve
+// Solves A_a^T U = (scale_m ? M R : R) along every line of axis AXIS.
+// Axis 2 (x): line l = blockIdx.x*CW + cl of the nz*ny x-lines.  Axes 1, 0:
+// columns i = blockIdx.x*CW + cl, blockIdx.y = k (y) or j (z).  A block has P
+// thread segments of SEG cells per line; P = blockDim.x / CW.  U must not
+// alias R.
+template <class V, int SEG, int AXIS>
+__global__ __launch_bounds__(256) void k_adj_line(Grid g, V view, int CW, const double *R,
+                                                  int scale_m, double *U) {
+  extern __shared__ double lds[];
+  view.load_tables(lds, AXIS);
+  const int P = blockDim.x / CW;
+  double *ra = lds + ((view.lds_doubles(AXIS) + 1) & ~1);
+  double *rc = ra + 2 * P * CW;
+  double *rr = rc + 2 * P * CW;
+  __syncthreads();
+
+  const int cl = threadIdx.x % CW, p = threadIdx.x / CW;
+  int L, stride, base;
+  bool valid;
+  if (AXIS == 2) {
+    const int nl = g.nz * g.ny, line = blockIdx.x * CW + cl;
+    valid = line < nl;
+    L = g.nx;
+    stride = 1;
+    base = (valid ? line : nl - 1) * g.nx;
+  } else {
+    const int i = blockIdx.x * CW + cl;
+    valid = i < g.nx;
+    const int ic = valid ? i : g.nx - 1;
+    L = AXIS == 1 ? g.ny : g.nz;
+    stride = AXIS == 1 ? g.nx : g.nx * g.ny;
+    base = AXIS == 1 ? blockIdx.y * g.ny * g.nx + ic : blockIdx.y * g.nx + ic;
+  }
+  const int t0 = p * SEG;
+
+  // entries of cells t0-1 .. t0+SEG (clamped into the line) and the RHS
+  typename V::Raw raw[SEG + 2];
+  double Rv[SEG];
+#pragma unroll
+  for (int q = 0; q < SEG + 2; q++)
+    raw[q] = view.fetch(AXIS, base + min(max(t0 + q - 1, 0), L - 1) * stride);
+#pragma unroll
+  for (int q = 0; q < SEG; q++) Rv[q] = R[base + min(t0 + q, L - 1) * stride];
+
+  double a[SEG], c[SEG], r[SEG];
+  Entry em = view.resolve(lds, raw[0]);
+  Entry ec = view.resolve(lds, raw[1]);
+#pragma unroll
+  for (int q = 0; q < SEG; q++) {
+    const int t = t0 + q;
+    const Entry ep = view.resolve(lds, raw[q + 2]);
+    const double gm = t >= 1 ? ec.g : 0.0;
+    const double gp = t + 1 <= L - 1 ? ep.g : 0.0;
+    const double b = ec.M + 0.5 * ec.w * (gm + gp);
+    const double inv = 1.0 / b;
+    const double rhs = scale_m ? ec.M * Rv[q] : Rv[q];
+    const bool ok = valid && t < L;
+    a[q] = ok ? -0.5 * em.w * gm * inv : 0.0;
+    c[q] = ok ? -0.5 * ep.w * gp * inv : 0.0;
+    r[q] = ok ? rhs * inv : 0.0;
+    em = ec;
+    ec = ep;
+  }
+  double As, Cs, Rs, Ae, Ce, Re;
+  segment_reduce<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
+  const int NR = 2 * P;
+  ra[(2 * p) * CW + cl] = As; rc[(2 * p) * CW + cl] = Cs; rr[(2 * p) * CW + cl] = Rs;
+  ra[(2 * p + 1) * CW + cl] = Ae; rc[(2 * p + 1) * CW + cl] = Ce; rr[(2 * p + 1) * CW + cl] = Re;
+  __syncthreads();
+  for (int d = 1; d < NR; d <<= 1) {
+    double na[2], nc[2], nr[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int row = 2 * p + h;
+      double aa = ra[row * CW + cl], cc = rc[row * CW + cl], r0 = rr[row * CW + cl];
+      const int rl = row - d >= 0 ? row - d : row, rh = row + d < NR ? row + d : row;
+      const double m0 = row - d >= 0 ? 1.0 : 0.0, m1 = row + d < NR ? 1.0 : 0.0;
+      const double aL = m0 * ra[rl * CW + cl], cL = m0 * rc[rl * CW + cl], rL = m0 * rr[rl * CW + cl];
+      const double aR = m1 * ra[rh * CW + cl], cR = m1 * rc[rh * CW + cl], rR = m1 * rr[rh * CW + cl];
+      pcr_combine(aa, cc, r0, aL, cL, rL, aR, cR, rR);
+      na[h] = aa; nc[h] = cc; nr[h] = r0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int row = 2 * p + h;
+      ra[row * CW + cl] = na[h]; rc[row * CW + cl] = nc[h]; rr[row * CW + cl] = nr[h];
+    }
+    __syncthreads();
+  }
+  const double xs = rr[(2 * p) * CW + cl], xe = rr[(2 * p + 1) * CW + cl];
+  if (!valid) return;
+  double x[SEG];
+  x[0] = xs;
+  x[SEG - 1] = xe;
+#pragma unroll
+  for (int q = 1; q < SEG - 1; q++) x[q] = r[q] - a[q] * xs - c[q] * xe;
+#pragma unroll
+  for (int q = 0; q < SEG; q++) {
+    const int t = t0 + q;
+    if (t < L) U[base + t * stride] = x[q];
+  }
+}
+
+// ------------------------------------------------------ adjoint stencil
+// One thread per cell.  Writes lam_out = dJ/dT^n (plus the readout adjoint
+// of this step's input field when ghist != null: the insulating-top readout
+// T0 - 5/16 (T1 - T0) of T^n), accumulates the sensitivities into gM and
+// gg[axis] (the cell's minus face on that axis), and writes gsrc = w u_x
+// when gsrc != null.
+template <class V>
+__global__ __launch_bounds__(256) void k_adj_stencil(Grid g, V view, AdjFields f) {
+  extern __shared__ double lds[];
+  view.load_tables(lds, -1);
+  __syncthreads();
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int sxy = nx * ny, n = sxy * nz;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int k = c / sxy, rem = c - k * sxy, j = rem / nx, i = rem - j * nx;
+  const int coord[3] = {k, j, i};
+  const int len[3] = {nz, ny, nx};
+  const int step[3] = {sxy, nx, 1};
+
+  const Entry e0 = view.resolve(view.table(lds, 0), view.fetch(0, c));
+  const double Mc = e0.M, wc = e0.w;
+  const double uxc = f.ux[c], uyc = f.uy[c], uzc = f.uz[c];
+  const double Tnc = f.Tn[c], Txc = f.Tx[c], Tyc = f.Ty[c], T1c = f.Tn1[c];
+
+  double lam = Mc * uxc;
+  // v^a: x: w u_x / 2, y: w (u_x - u_y / 2), z: w (u_x - u_z / 2)
+  auto vof = [](int a, double w, double ux, double uy, double uz) {
+    return a == 2 ? 0.5 * w * ux : a == 1 ? w * (ux - 0.5 * uy) : w * (ux - 0.5 * uz);
+  };
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const double vc = vof(a, wc, uxc, uyc, uzc);
+    if (coord[a] > 0) {
+      const int m = c - step[a];
+      const Entry ec = view.resolve(view.table(lds, a), view.fetch(a, c));
+      const Entry emn = view.resolve(view.table(lds, a), view.fetch(a, m));
+      const double gf = ec.g, wm = emn.w;
+      const double uxm = f.ux[m], uym = f.uy[m], uzm = f.uz[m];
+      lam += gf * (vof(a, wm, uxm, uym, uzm) - vc);
+      const double Tnm = f.Tn[m];
+      const double Ex = wm * uxm - wc * uxc;
+      double gg;
+      if (a == 2) {
+        gg = 0.5 * ((Txc + Tnc) - (f.Tx[m] + Tnm)) * Ex;
+      } else if (a == 1) {
+        const double Ey = wm * uym - wc * uyc;
+        gg = (Tnc - Tnm) * Ex + 0.5 * ((Tyc - Tnc) - (f.Ty[m] - Tnm)) * Ey;
+      } else {
+        const double Ez = wm * uzm - wc * uzc;
+        gg = (Tnc - Tnm) * Ex + 0.5 * ((T1c - Tnc) - (f.Tn1[m] - Tnm)) * Ez;
+      }
+      f.gg[a][c] += gg;
+    }
+    if (coord[a] < len[a] - 1) {
+      const int pn = c + step[a];
+      const Entry ep = view.resolve(view.table(lds, a), view.fetch(a, pn));
+      lam += ep.g * (vof(a, ep.w, f.ux[pn], f.uy[pn], f.uz[pn]) - vc);
+    }
+  }
+  if (f.ghist) {
+    if (k == 0) lam += (21.0 / 16.0) * f.ghist[rem];
+    else if (k == 1) lam -= (5.0 / 16.0) * f.ghist[rem];
+  }
+  f.lam_out[c] = lam;
+  f.gM[c] -= uxc * (Txc - Tnc) + uyc * (Tyc - Txc) + uzc * (T1c - Tyc);
+  if (f.gsrc) f.gsrc[c] = wc * uxc;
+}
+
+// lam = readout adjoint of ghist (ny, nx) on the top two layers, 0 elsewhere
+__global__ void k_lam_from_readout(Grid g, const double *ghist, double *lam) {
+  const int sxy = g.nx * g.ny, n = sxy * g.nz;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const int k = c / sxy, rem = c - k * sxy;
+    lam[c] = k == 0 ? (21.0 / 16.0) * ghist[rem] : k == 1 ? -(5.0 / 16.0) * ghist[rem] : 0.0;
+  }
+}
+
+__global__ void k_fill(double *a, size_t n, double v) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x)
+    a[t] = v;
+}
+
+// ---------------------------------------------------------- launchers
+AdjLineConfig adj_line_config(int L) {
+  AdjLineConfig c;
+  c.seg = L <= 64 ? 4 : (L <= 512 ? 8 : 16);
+  const int P = (L + c.seg - 1) / c.seg;
+  c.cw = 64;
+  while (c.cw > 1 && P * c.cw > 256) c.cw >>= 1;
+  if (P * c.cw > 256) c.seg = 0;  // unsupported length
+  return c;
+}
+
+template <class V, int SEG, int AXIS>
+static hipError_t launch_adj_line_t(const Grid &g, const V &v, int CW, const double *R,
+                                    int scale_m, double *U, hipStream_t s) {
+  const int L = AXIS == 2 ? g.nx : AXIS == 1 ? g.ny : g.nz;
+  const int P = (L + SEG - 1) / SEG;
+  dim3 grid;
+  if (AXIS == 2) grid = dim3((g.nz * g.ny + CW - 1) / CW, 1);
+  else grid = dim3((g.nx + CW - 1) / CW, AXIS == 1 ? g.nz : g.ny);
+  const size_t lds = sizeof(double) * ((size_t)((v.lds_doubles(AXIS) + 1) & ~1) + 6 * (size_t)P * CW);
+  hipLaunchKernelGGL((k_adj_line<V, SEG, AXIS>), grid, dim3(P * CW), lds, s, g, v, CW, R, scale_m,
+                     U);
+  return hipGetLastError();
+}
+
+template <class V, int SEG>
+static hipError_t launch_adj_line_s(const Grid &g, const V &v, int axis, int CW, const double *R,
+                                    int scale_m, double *U, hipStream_t s) {
+  switch (axis) {
+    case 0: return launch_adj_line_t<V, SEG, 0>(g, v, CW, R, scale_m, U, s);
+    case 1: return launch_adj_line_t<V, SEG, 1>(g, v, CW, R, scale_m, U, s);
+    case 2: return launch_adj_line_t<V, SEG, 2>(g, v, CW, R, scale_m, U, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class V>
+static hipError_t launch_adj_line_v(const Grid &g, const V &v, int axis, const double *R,
+                                    int scale_m, double *U, hipStream_t s) {
+  const int L = axis == 2 ? g.nx : axis == 1 ? g.ny : g.nz;
+  const AdjLineConfig c = adj_line_config(L);
+  switch (c.seg) {
+    case 4: return launch_adj_line_s<V, 4>(g, v, axis, c.cw, R, scale_m, U, s);
+    case 8: return launch_adj_line_s<V, 8>(g, v, axis, c.cw, R, scale_m, U, s);
+    case 16: return launch_adj_line_s<V, 16>(g, v, axis, c.cw, R, scale_m, U, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_adj_line(const Grid &g, const ClassView &v, int axis, const double *R,
+                           int scale_m, double *U, hipStream_t s) {
+  return launch_adj_line_v(g, v, axis, R, scale_m, U, s);
+}
+hipError_t launch_adj_line(const Grid &g, const DenseView &v, int axis, const double *R,
+                           int scale_m, double *U, hipStream_t s) {
+  return launch_adj_line_v(g, v, axis, R, scale_m, U, s);
+}
+
+template <class V>
+static hipError_t launch_adj_stencil_t(const Grid &g, const V &v, const AdjFields &f,
+                                       hipStream_t s) {
+  const long n = (long)g.nx * g.ny * g.nz;
+  const size_t lds = sizeof(double) * (size_t)v.lds_doubles(-1);
+  hipLaunchKernelGGL((k_adj_stencil<V>), dim3((unsigned)((n + 255) / 256)), dim3(256), lds, s, g, v,
+                     f);
+  return hipGetLastError();
+}
+hipError_t launch_adj_stencil(const Grid &g, const ClassView &v, const AdjFields &f,
+                              hipStream_t s) {
+  return launch_adj_stencil_t(g, v, f, s);
+}
+hipError_t launch_adj_stencil(const Grid &g, const DenseView &v, const AdjFields &f,
+                              hipStream_t s) {
+  return launch_adj_stencil_t(g, v, f, s);
+}
+
+hipError_t launch_lam_from_readout(const Grid &g, const double *ghist, double *lam,
+                                   hipStream_t s) {
+  hipLaunchKernelGGL(k_lam_from_readout, dim3(1024), dim3(256), 0, s, g, ghist, lam);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(double *a, size_t n, double v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, a, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace hs2

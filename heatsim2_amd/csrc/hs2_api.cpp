@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/heatsim2_amd.h"
+#include "hs2_adjoint.h"
 #include "hs2_launch.h"
 
 namespace {
@@ -175,6 +176,42 @@ int run_step(hs2_plan *p, const double *Tin, double *Tout, const uint8_t *vmap,
   HIP_TRY(sweep_x(p, Tin, p->d_ws, vmap, d_vtab, sfield, s));
   HIP_TRY(sweep_s(p, 1, Tin, p->d_ws, p->d_ws, nullptr, s));
   HIP_TRY(sweep_s(p, 0, Tin, p->d_ws, Tout, surf, s));
+  return HS2_OK;
+}
+
+// one step writing its intermediate states: T* -> Tx, T** -> Ty, T^{n+1} -> Tout
+int step_states(hs2_plan *p, const double *Tin, double *Tx, double *Ty, double *Tout,
+                const double *sfield, double *surf, hipStream_t s) {
+  HIP_TRY(sweep_x(p, Tin, Tx, nullptr, nullptr, sfield, s));
+  HIP_TRY(sweep_s(p, 1, Tin, Tx, Ty, nullptr, s));
+  HIP_TRY(sweep_s(p, 0, Tin, Ty, Tout, surf, s));
+  return HS2_OK;
+}
+
+hipError_t adj_line(const hs2_plan *p, int axis, const double *R, int scale_m, double *U,
+                    hipStream_t s) {
+  return p->mode == MODE_CLASS ? hs2::launch_adj_line(grid_of(p), p->cv, axis, R, scale_m, U, s)
+                               : hs2::launch_adj_line(grid_of(p), p->dv, axis, R, scale_m, U, s);
+}
+
+hipError_t adj_stencil(const hs2_plan *p, const hs2::AdjFields &f, hipStream_t s) {
+  return p->mode == MODE_CLASS ? hs2::launch_adj_stencil(grid_of(p), p->cv, f, s)
+                               : hs2::launch_adj_stencil(grid_of(p), p->dv, f, s);
+}
+
+// step -> source index (-1: none); validates the schedule
+int source_schedule(int nsteps, const int32_t *h_src_step, int nsrc, const double *d_src,
+                    std::vector<int> *out, const char *who) {
+  out->assign((size_t)nsteps, -1);
+  if (nsrc < 0 || (nsrc > 0 && (!h_src_step || !d_src)))
+    return fail(HS2_E_ARG, "%s: bad source arguments", who);
+  if (nsrc > 0 && !aligned16(d_src)) return fail(HS2_E_ARG, "%s: d_src must be 16-byte aligned", who);
+  for (int s = 0; s < nsrc; s++) {
+    const int st = h_src_step[s];
+    if (st < 0 || st >= nsteps) return fail(HS2_E_ARG, "%s: source %d at step %d outside [0,%d)", who, s, st, nsteps);
+    if ((*out)[st] >= 0) return fail(HS2_E_ARG, "%s: two sources at step %d", who, st);
+    (*out)[st] = s;
+  }
   return HS2_OK;
 }
 
@@ -549,6 +586,107 @@ int hs2_run_timed(hs2_plan *p, double *d_T, int nsteps, double *kernel_ms, void 
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   return rc;
+}
+
+size_t hs2_grad_work_doubles(const hs2_plan *p, int ckpt_every) {
+  if (!p || ckpt_every < 1) return 0;
+  return ((size_t)ckpt_every + 8) * (size_t)p->n;
+}
+
+int hs2_grad_forward(hs2_plan *p, double *d_T, int nsteps, int ckpt_every, double *d_ckpt,
+                     const double *d_src, const int32_t *h_src_step, int nsrc,
+                     double *d_surf_hist, void *stream) {
+  if (!p || !d_T || !d_ckpt || nsteps < 1 || ckpt_every < 1 || !aligned16(d_T) ||
+      !aligned16(d_ckpt))
+    return fail(HS2_E_ARG, "hs2_grad_forward: bad argument");
+  if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_grad_forward: coefficients not set");
+  if (d_surf_hist && p->grid.nz < 2)
+    return fail(HS2_E_ARG, "hs2_grad_forward: the surface readout needs nz >= 2");
+  std::vector<int> src;
+  int rc = source_schedule(nsteps, h_src_step, nsrc, d_src, &src, "hs2_grad_forward");
+  if (rc != HS2_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)p->n, nyx = (size_t)p->grid.ny * p->grid.nx;
+  for (int t = 0; t < nsteps; t++) {
+    if (t % ckpt_every == 0)
+      HIP_TRY(hipMemcpyAsync(d_ckpt + (size_t)(t / ckpt_every) * n, d_T, n * sizeof(double),
+                             hipMemcpyDeviceToDevice, s));
+    const double *sf = src[t] >= 0 ? d_src + (size_t)src[t] * n : nullptr;
+    rc = run_step(p, d_T, d_T, nullptr, nullptr, sf, d_surf_hist ? d_surf_hist + t * nyx : nullptr, s);
+    if (rc != HS2_OK) return rc;
+  }
+  return HS2_OK;
+}
+
+int hs2_grad_backward(hs2_plan *p, int nsteps, int ckpt_every, const double *d_ckpt,
+                      const double *d_src, const int32_t *h_src_step, int nsrc,
+                      const double *d_grad_hist, double *d_gM, double *d_gz, double *d_gy,
+                      double *d_gx, double *d_gsrc, double *d_gT0, double *d_work,
+                      size_t work_doubles, void *stream) {
+  if (!p || !d_ckpt || !d_grad_hist || !d_gM || !d_gz || !d_gy || !d_gx || !d_work ||
+      nsteps < 1 || ckpt_every < 1 || (nsrc > 0 && !d_gsrc))
+    return fail(HS2_E_ARG, "hs2_grad_backward: bad argument");
+  if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_grad_backward: coefficients not set");
+  if (p->grid.nz < 2) return fail(HS2_E_ARG, "hs2_grad_backward: the surface readout needs nz >= 2");
+  if (!aligned16(d_ckpt) || !aligned16(d_work))
+    return fail(HS2_E_ARG, "hs2_grad_backward: d_ckpt and d_work must be 16-byte aligned");
+  const size_t need = hs2_grad_work_doubles(p, ckpt_every);
+  if (work_doubles < need)
+    return fail(HS2_E_ARG, "hs2_grad_backward: work buffer has %zu doubles, needs %zu",
+                work_doubles, need);
+  const hs2::Grid g = grid_of(p);
+  if (!hs2::adj_line_config(g.nx).seg || !hs2::adj_line_config(g.ny).seg ||
+      !hs2::adj_line_config(g.nz).seg)
+    return fail(HS2_E_UNSUPPORTED, "hs2_grad_backward: axis length above 1024");
+  std::vector<int> src;
+  int rc = source_schedule(nsteps, h_src_step, nsrc, d_src, &src, "hs2_grad_backward");
+  if (rc != HS2_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)p->n, nyx = (size_t)g.ny * g.nx;
+  const int K = ckpt_every;
+  double *seg = d_work;  // K fields: T^{n0} .. T^{n0+K-1}
+  double *Tx = seg + (size_t)K * n, *Ty = Tx + n, *Tn1 = Ty + n;
+  double *ux = Tn1 + n, *uy = ux + n, *uz = uy + n;
+  double *lam = uz + n, *lam2 = lam + n;
+  HIP_TRY(hs2::launch_fill(d_gM, n, 0.0, s));
+  HIP_TRY(hs2::launch_fill(d_gz, n, 0.0, s));
+  HIP_TRY(hs2::launch_fill(d_gy, n, 0.0, s));
+  HIP_TRY(hs2::launch_fill(d_gx, n, 0.0, s));
+  HIP_TRY(hs2::launch_lam_from_readout(g, d_grad_hist + (size_t)(nsteps - 1) * nyx, lam, s));
+  const int nseg = (nsteps + K - 1) / K;
+  for (int sg = nseg - 1; sg >= 0; sg--) {
+    const int n0 = sg * K, n1 = std::min(nsteps, n0 + K);
+    HIP_TRY(hipMemcpyAsync(seg, d_ckpt + (size_t)sg * n, n * sizeof(double),
+                           hipMemcpyDeviceToDevice, s));
+    for (int t = n0; t < n1 - 1; t++) {  // recompute the segment's states
+      const double *sf = src[t] >= 0 ? d_src + (size_t)src[t] * n : nullptr;
+      rc = step_states(p, seg + (size_t)(t - n0) * n, Tx, Ty, seg + (size_t)(t - n0 + 1) * n, sf,
+                       nullptr, s);
+      if (rc != HS2_OK) return rc;
+    }
+    for (int t = n1 - 1; t >= n0; t--) {
+      const double *Tn = seg + (size_t)(t - n0) * n;
+      const double *sf = src[t] >= 0 ? d_src + (size_t)src[t] * n : nullptr;
+      rc = step_states(p, Tn, Tx, Ty, Tn1, sf, nullptr, s);
+      if (rc != HS2_OK) return rc;
+      HIP_TRY(adj_line(p, 0, lam, 0, uz, s));
+      HIP_TRY(adj_line(p, 1, uz, 1, uy, s));
+      HIP_TRY(adj_line(p, 2, uy, 1, ux, s));
+      hs2::AdjFields f{};
+      f.Tn = Tn; f.Tx = Tx; f.Ty = Ty; f.Tn1 = Tn1;
+      f.ux = ux; f.uy = uy; f.uz = uz;
+      f.ghist = t >= 1 ? d_grad_hist + (size_t)(t - 1) * nyx : nullptr;
+      f.lam_out = lam2;
+      f.gM = d_gM;
+      f.gg[0] = d_gz; f.gg[1] = d_gy; f.gg[2] = d_gx;
+      f.gsrc = src[t] >= 0 ? d_gsrc + (size_t)src[t] * n : nullptr;
+      HIP_TRY(adj_stencil(p, f, s));
+      std::swap(lam, lam2);
+    }
+  }
+  if (d_gT0)
+    HIP_TRY(hipMemcpyAsync(d_gT0, lam, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return HS2_OK;
 }
 
 int hs2_surface_temperature(const double *d_T, int nz, int ny, int nx, double dz, double *d_out,

@@ -18,6 +18,7 @@
 #ifndef HEATSIM2_AMD_H
 #define HEATSIM2_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -130,6 +131,43 @@ int hs2_run(hs2_plan *plan, double *d_T, int nsteps, const uint8_t *d_vmap,
  * the mean device time per launch of the x, y and z sweep.  A measurement
  * entry point with no upstream counterpart (the upstream has no timers). */
 int hs2_run_timed(hs2_plan *plan, double *d_T, int nsteps, double *kernel_ms, void *stream);
+
+/* ---- Gradients (no upstream counterpart: heatsim2 has no adjoint).
+ * The differentiable time loop runs nsteps source-driven steps
+ *   T^{n+1} = step(T^n; M, g, S^n),   hist[n] = readout(T^{n+1})
+ * (readout: the insulating-top surface temperature of
+ * surface_temperature.py:31-36), where S^n is the source field d_src[s]
+ * (n doubles, W m^-3) of the source s with h_src_step[s] == n (at most one
+ * per step), or none.  Given dJ/dhist, hs2_grad_backward returns dJ/dM,
+ * dJ/dg per axis (for each cell's minus face, as in hs2_plan_set_dense),
+ * dJ/dS for every source and dJ/dT^0, through a hand-written adjoint step:
+ * transposed line solves in reverse axis order plus the adjoint stencil,
+ * with the forward states recomputed from checkpoints taken every
+ * ckpt_every steps.  Works in both coefficient modes; gradients with
+ * respect to per-voxel parameters need hs2_plan_set_dense. */
+
+/* Doubles of device workspace hs2_grad_backward needs: (ckpt_every+8)*n. */
+size_t hs2_grad_work_doubles(const hs2_plan *plan, int ckpt_every);
+
+/* Forward pass: nsteps steps in place on d_T (T^0 in, T^nsteps out),
+ * writing T^{k*ckpt_every} to d_ckpt + k*n (ceil(nsteps/ckpt_every) fields)
+ * and, if d_surf_hist is not NULL, the readout after every step
+ * (nsteps x ny x nx). */
+int hs2_grad_forward(hs2_plan *plan, double *d_T, int nsteps, int ckpt_every, double *d_ckpt,
+                     const double *d_src, const int32_t *h_src_step, int nsrc,
+                     double *d_surf_hist, void *stream);
+
+/* Backward pass over the run that hs2_grad_forward recorded in d_ckpt (same
+ * plan coefficients, nsteps, ckpt_every and sources).  d_grad_hist: dJ/dhist
+ * (nsteps x ny x nx).  Writes d_gM, d_gz, d_gy, d_gx (n each), d_gsrc
+ * (nsrc x n; may be NULL when nsrc == 0) and, if not NULL, d_gT0 (n).
+ * d_work: hs2_grad_work_doubles() doubles of scratch.  Needs nz >= 2 and
+ * every axis length <= 1024. */
+int hs2_grad_backward(hs2_plan *plan, int nsteps, int ckpt_every, const double *d_ckpt,
+                      const double *d_src, const int32_t *h_src_step, int nsrc,
+                      const double *d_grad_hist, double *d_gM, double *d_gz, double *d_gy,
+                      double *d_gx, double *d_gsrc, double *d_gT0, double *d_work,
+                      size_t work_doubles, void *stream);
 
 /* Insulating-top surface readout of a (nz,ny,nx) field into (ny,nx):
  * T0 - 5/16 (T1 - T0).  Replaces insulating_z_min_surface_temperature

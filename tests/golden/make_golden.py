@@ -177,7 +177,7 @@ def case_grad_tiny(seed=11):
                 h=300.0, obs_noise=obs_noise)
 
 
-def grad_loss_ref(hs, g, kz, ky, kx, rhoc, E, obs):
+def grad_loss_ref(hs, g, kz, ky, kx, rhoc, E, obs, T0=0.0):
     nz, ny, nx = g["shape"]
     dz, dy, dx = g["d"]
     m, bz, by, bx, v = g["maps"]
@@ -193,7 +193,7 @@ def grad_loss_ref(hs, g, kz, ky, kx, rhoc, E, obs):
     with _refload.quiet():
         P, S = hs.setup(0.0, 0.0, 0.0, dz, dy, dx, nz, ny, nx, g["dt"], mats, bnds, vol,
                         m, bz, by, bx, v)
-    T = np.full((nz, ny, nx), 293.0)
+    T = np.full((nz, ny, nx), T0)
     from_st = __import__("heatsim2.surface_temperature", fromlist=["x"])
     hist = []
     for n in range(g["nsteps"]):
@@ -205,7 +205,12 @@ def grad_loss_ref(hs, g, kz, ky, kx, rhoc, E, obs):
     return float(np.sum((hist - obs) ** 2)), hist
 
 
-def gen_grad_case(hs, nvox=10, rel=1e-5):
+def gen_grad_case(hs, nvox=10, rel=3e-3):
+    """Finite differences of the upstream solver's loss.  Temperatures are
+    relative to ambient (T0 = 0): the step is affine in T, so this changes
+    no gradient but keeps round-off in T at the scale of the rise rather
+    than of 293 K.  Each derivative is a Richardson-extrapolated central
+    difference, (4 D(h) - D(2h)) / 3 with h = rel * |value|."""
     g = case_grad_tiny()
     base = dict(kz=g["kz"], ky=g["ky"], kx=g["kx"], rhoc=g["rhoc"])
     # observations: the forward history at a perturbed "true" field plus noise
@@ -217,27 +222,32 @@ def gen_grad_case(hs, nvox=10, rel=1e-5):
     n = g["kz"].size
     out = dict(shape=np.array(g["shape"]), d=np.array(g["d"]), dt=np.array(g["dt"]),
                nsteps=np.array(g["nsteps"]), h=np.array(g["h"]), E=np.array(g["E"]),
+               T0=np.array(0.0),
                material_elements=g["maps"][0], boundary_z_elements=g["maps"][1],
                boundary_y_elements=g["maps"][2], boundary_x_elements=g["maps"][3],
                volumetric_elements=g["maps"][4], obs=obs, loss=np.array(L0), hist=hist0,
                fd_rel_step=np.array(rel), **base)
+
+    def richardson(f, x0):
+        h = rel * abs(x0)
+        d1 = (f(x0 + h) - f(x0 - h)) / (2 * h)
+        d2 = (f(x0 + 2 * h) - f(x0 - 2 * h)) / (4 * h)
+        return (4 * d1 - d2) / 3
+
     for name in ("kz", "ky", "kx", "rhoc"):
         vox = rng.choice(n, size=nvox, replace=False)
         fd = []
         for idx in vox:
-            vals = []
-            for sgn in (+1, -1):
+            def f(val):
                 p = {k: v.copy() for k, v in base.items()}
-                step = rel * p[name].flat[idx]
-                p[name].flat[idx] += sgn * step
-                vals.append(grad_loss_ref(hs, g, p["kz"], p["ky"], p["kx"], p["rhoc"], g["E"], obs)[0])
-            fd.append((vals[0] - vals[1]) / (2 * step))
+                p[name].flat[idx] = val
+                return grad_loss_ref(hs, g, p["kz"], p["ky"], p["kx"], p["rhoc"], g["E"], obs)[0]
+            fd.append(richardson(f, base[name].flat[idx]))
         out["fd_%s_vox" % name] = vox.astype(np.int64)
         out["fd_%s" % name] = np.array(fd)
-    stepE = rel * g["E"]
-    Lp = grad_loss_ref(hs, g, base["kz"], base["ky"], base["kx"], base["rhoc"], g["E"] + stepE, obs)[0]
-    Lm = grad_loss_ref(hs, g, base["kz"], base["ky"], base["kx"], base["rhoc"], g["E"] - stepE, obs)[0]
-    out["fd_E"] = np.array((Lp - Lm) / (2 * stepE))
+    out["fd_E"] = np.array(richardson(
+        lambda e: grad_loss_ref(hs, g, base["kz"], base["ky"], base["kx"], base["rhoc"], e, obs)[0],
+        g["E"]))
     return out
 
 
