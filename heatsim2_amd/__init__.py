@@ -23,6 +23,7 @@ import heatsim2_amd.crank_nicolson as crank_nicolson
 import heatsim2_amd.alternatingdirection as alternatingdirection
 import heatsim2_amd.autotune as autotune
 import heatsim2_amd.adjoint as adjoint
+import heatsim2_amd.batch as batch
 
 __version__ = "0.1.0"
 

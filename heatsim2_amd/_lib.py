@@ -33,6 +33,8 @@ class HS2Error(RuntimeError):
 EXPORTS = ("hs2_plan_create", "hs2_plan_destroy", "hs2_plan_set_compact", "hs2_plan_set_dense",
            "hs2_plan_expand", "hs2_plan_set_config", "hs2_plan_info", "hs2_step", "hs2_run",
            "hs2_run_timed", "hs2_grad_work_doubles", "hs2_grad_forward", "hs2_grad_backward",
+           "hs2_batch_create", "hs2_batch_destroy", "hs2_batch_layout", "hs2_batch_set_dense",
+           "hs2_batch_set_options", "hs2_batch_forward", "hs2_batch_backward",
            "hs2_surface_temperature", "hs2_last_error", "hs2_version")
 
 _lib = None
@@ -75,6 +77,20 @@ def load():
     L.hs2_grad_backward.argtypes = [vp, i32, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp,
                                     ctypes.c_size_t, vp]
     L.hs2_grad_backward.restype = i32
+    L.hs2_batch_create.argtypes = [vp, i32, i32, ctypes.POINTER(vp)]
+    L.hs2_batch_create.restype = i32
+    L.hs2_batch_destroy.argtypes = [vp]
+    L.hs2_batch_destroy.restype = None
+    L.hs2_batch_layout.argtypes = [vp, vp, vp, vp]
+    L.hs2_batch_layout.restype = i32
+    L.hs2_batch_set_dense.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.hs2_batch_set_dense.restype = i32
+    L.hs2_batch_set_options.argtypes = [vp, i32, i32]
+    L.hs2_batch_set_options.restype = i32
+    L.hs2_batch_forward.argtypes = [vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    L.hs2_batch_forward.restype = i32
+    L.hs2_batch_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    L.hs2_batch_backward.restype = i32
     L.hs2_surface_temperature.argtypes = [vp, i32, i32, i32, dbl, vp, vp]
     L.hs2_surface_temperature.restype = i32
     L.hs2_last_error.argtypes = []

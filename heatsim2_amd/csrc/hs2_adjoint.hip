@@ -29,18 +29,19 @@
 #include <algorithm>
 
 #include "hs2_adjoint.h"
+#include "hs2_batch.h"
 
 namespace hs2 {
 
 // ------------------------------------------------- transposed line solve
 // Solves A_a^T U = (scale_m ? M R : R) along every line of axis AXIS.
-// Axis 2 (x): line l = blockIdx.x*CW + cl of the nz*ny x-lines.  Axes 1, 0:
-// columns i = blockIdx.x*CW + cl, blockIdx.y = k (y) or j (z).  A block has P
+// Block (bx, by).  Axis 2 (x): line l = bx*CW + cl of the nz*ny x-lines.
+// Axes 1, 0: columns i = bx*CW + cl, by = k (y) or j (z).  A block has P
 // thread segments of SEG cells per line; P = blockDim.x / CW.  U must not
 // alias R.
 template <class V, int SEG, int AXIS>
-__global__ __launch_bounds__(256) void k_adj_line(Grid g, V view, int CW, const double *R,
-                                                  int scale_m, double *U) {
+__device__ __forceinline__ void adj_line_body(Grid g, V view, int CW, const double *R,
+                                              int scale_m, double *U, int bx, int by) {
   extern __shared__ double lds[];
   view.load_tables(lds, AXIS);
   const int P = blockDim.x / CW;
@@ -53,18 +54,18 @@ __global__ __launch_bounds__(256) void k_adj_line(Grid g, V view, int CW, const 
   int L, stride, base;
   bool valid;
   if (AXIS == 2) {
-    const int nl = g.nz * g.ny, line = blockIdx.x * CW + cl;
+    const int nl = g.nz * g.ny, line = bx * CW + cl;
     valid = line < nl;
     L = g.nx;
     stride = 1;
     base = (valid ? line : nl - 1) * g.nx;
   } else {
-    const int i = blockIdx.x * CW + cl;
+    const int i = bx * CW + cl;
     valid = i < g.nx;
     const int ic = valid ? i : g.nx - 1;
     L = AXIS == 1 ? g.ny : g.nz;
     stride = AXIS == 1 ? g.nx : g.nx * g.ny;
-    base = AXIS == 1 ? blockIdx.y * g.ny * g.nx + ic : blockIdx.y * g.nx + ic;
+    base = AXIS == 1 ? by * g.ny * g.nx + ic : by * g.nx + ic;
   }
   const int t0 = p * SEG;
 
@@ -137,6 +138,12 @@ __global__ __launch_bounds__(256) void k_adj_line(Grid g, V view, int CW, const 
   }
 }
 
+template <class V, int SEG, int AXIS>
+__global__ __launch_bounds__(256) void k_adj_line(Grid g, V view, int CW, const double *R,
+                                                  int scale_m, double *U) {
+  adj_line_body<V, SEG, AXIS>(g, view, CW, R, scale_m, U, blockIdx.x, blockIdx.y);
+}
+
 // ------------------------------------------------------ adjoint stencil
 // One thread per cell.  Writes lam_out = dJ/dT^n (plus the readout adjoint
 // of this step's input field when ghist != null: the insulating-top readout
@@ -144,13 +151,13 @@ __global__ __launch_bounds__(256) void k_adj_line(Grid g, V view, int CW, const 
 // gg[axis] (the cell's minus face on that axis), and writes gsrc = w u_x
 // when gsrc != null.
 template <class V>
-__global__ __launch_bounds__(256) void k_adj_stencil(Grid g, V view, AdjFields f) {
+__device__ __forceinline__ void adj_stencil_body(Grid g, V view, const AdjFields &f, int blk) {
   extern __shared__ double lds[];
   view.load_tables(lds, -1);
   __syncthreads();
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int sxy = nx * ny, n = sxy * nz;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blk * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int k = c / sxy, rem = c - k * sxy, j = rem / nx, i = rem - j * nx;
   const int coord[3] = {k, j, i};
@@ -204,6 +211,11 @@ __global__ __launch_bounds__(256) void k_adj_stencil(Grid g, V view, AdjFields f
   f.lam_out[c] = lam;
   f.gM[c] -= uxc * (Txc - Tnc) + uyc * (Tyc - Txc) + uzc * (T1c - Tyc);
   if (f.gsrc) f.gsrc[c] = wc * uxc;
+}
+
+template <class V>
+__global__ __launch_bounds__(256) void k_adj_stencil(Grid g, V view, AdjFields f) {
+  adj_stencil_body<V>(g, view, f, blockIdx.x);
 }
 
 // lam = readout adjoint of ghist (ny, nx) on the top two layers, 0 elsewhere
@@ -307,6 +319,130 @@ hipError_t launch_fill(double *a, size_t n, double v, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, a, n, v);
+  return hipGetLastError();
+}
+
+
+// ------------------------------------------------------ batched adjoint
+template <int SEG, int AXIS>
+__global__ __launch_bounds__(kBatchCW *kBatchMaxLen / kBatchSeg) void kb_adj_line(
+    BatchMap m, DenseView v, const double *R, int scale_m, double *U) {
+  BatchInst I;
+  int local, t;
+  if (!batch_locate(m, I, local, t)) return;
+  int bx = local, by = 0;
+  if (AXIS != 2) {
+    const int nbx = (I.nx + kBatchCW - 1) / kBatchCW;
+    bx = local % nbx;
+    by = local / nbx;
+  }
+  adj_line_body<DenseView, SEG, AXIS>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), kBatchCW,
+                                      R + I.off, scale_m, U + I.off, bx, by);
+}
+
+// f: base pointers of the concatenated fields; f.ghist: base of the
+// concatenated dJ/dhist; f.gsrc: base of the concatenated source gradients.
+__global__ __launch_bounds__(256) void kb_adj_stencil(BatchMap m, DenseView v, AdjFields f) {
+  BatchInst I;
+  int local, t;
+  if (!batch_locate(m, I, local, t)) return;
+  const long long o = I.off;
+  AdjFields g = f;
+  g.Tn += o; g.Tx += o; g.Ty += o; g.Tn1 += o;
+  g.ux += o; g.uy += o; g.uz += o;
+  g.lam_out += o; g.gM += o;
+  g.gg[0] += o; g.gg[1] += o; g.gg[2] += o;
+  g.ghist = (f.ghist && t >= 1) ? f.ghist + I.hoff + (long long)(t - 1) * I.ny * I.nx : nullptr;
+  g.gsrc = (f.gsrc && t == I.src_step) ? f.gsrc + o : nullptr;
+  adj_stencil_body<DenseView>(Grid{I.nz, I.ny, I.nx}, offset_view(v, o), g, local);
+}
+
+// forward: before step t, T^t -> checkpoint slot ceil((n-t)/K)-1 when
+// (n - t) % K == 0 or t == 0
+__global__ void kb_ckpt_store(BatchMap m, const double *T, double *ck) {
+  BatchInst I;
+  int local, t;
+  if (!batch_locate(m, I, local, t)) return;
+  const int r = I.nsteps - t;
+  if (r % m.K != 0 && t != 0) return;
+  const long long n = (long long)I.nz * I.ny * I.nx;
+  const long long c = (long long)local * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int slot = (r + m.K - 1) / m.K - 1;
+  ck[I.ckoff + slot * n + c] = T[I.off + c];
+}
+
+// backward segment start: checkpoint of the segment -> segment slot K - (e - s)
+__global__ void kb_ckpt_load(BatchMap m, const double *ck, double *seg, long long N) {
+  BatchInst I;
+  int local, t;
+  if (!batch_locate(m, I, local, t)) return;  // RULE_SEG: t = segment start s
+  const long long n = (long long)I.nz * I.ny * I.nx;
+  const long long c = (long long)local * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int e = I.nsteps - m.ctr[1] * m.K;
+  const int slot = m.K - (e - t);
+  ck += I.ckoff + (long long)m.ctr[1] * n;
+  seg[(long long)slot * N + I.off + c] = ck[c];
+}
+
+// lam = readout adjoint of the last history row of each instance
+__global__ void kb_lam_init(BatchMap m, const double *ghist, double *lam) {
+  BatchInst I;
+  int local, t;
+  if (!batch_locate(m, I, local, t)) return;
+  const long long nyx = (long long)I.ny * I.nx, n = nyx * I.nz;
+  const long long c = (long long)local * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const long long k = c / nyx, rem = c - k * nyx;
+  const double *gh = ghist + I.hoff + (long long)(I.nsteps - 1) * nyx;
+  lam[I.off + c] = k == 0 ? (21.0 / 16.0) * gh[rem] : k == 1 ? -(5.0 / 16.0) * gh[rem] : 0.0;
+}
+
+// ctr[which] = value (set) or += value (add); one thread
+__global__ void kb_counter(int *ctr, int which, int value, int add) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctr[which] = add ? ctr[which] + value : value;
+}
+
+hipError_t launch_batch_adj_line(const BatchMap &m, int nblocks, int axis, const DenseView &v,
+                                 const double *R, int scale_m, double *U, hipStream_t s) {
+  constexpr int P = kBatchMaxLen / kBatchSeg;
+  const size_t shm = sizeof(double) * 6 * (size_t)P * kBatchCW;
+  const dim3 grid(nblocks), block(P * kBatchCW);
+  switch (axis) {
+    case 0: hipLaunchKernelGGL((kb_adj_line<kBatchSeg, 0>), grid, block, shm, s, m, v, R, scale_m, U); break;
+    case 1: hipLaunchKernelGGL((kb_adj_line<kBatchSeg, 1>), grid, block, shm, s, m, v, R, scale_m, U); break;
+    default: hipLaunchKernelGGL((kb_adj_line<kBatchSeg, 2>), grid, block, shm, s, m, v, R, scale_m, U); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_adj_stencil(const BatchMap &m, int nblocks, const DenseView &v,
+                                    const AdjFields &f, hipStream_t s) {
+  hipLaunchKernelGGL(kb_adj_stencil, dim3(nblocks), dim3(256), 0, s, m, v, f);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_ckpt_store(const BatchMap &m, int nblocks, const double *T, double *ck,
+                                   hipStream_t s) {
+  hipLaunchKernelGGL(kb_ckpt_store, dim3(nblocks), dim3(256), 0, s, m, T, ck);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_ckpt_load(const BatchMap &m, int nblocks, const double *ck, double *seg,
+                                  long long N, hipStream_t s) {
+  hipLaunchKernelGGL(kb_ckpt_load, dim3(nblocks), dim3(256), 0, s, m, ck, seg, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_lam_init(const BatchMap &m, int nblocks, const double *ghist, double *lam,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(kb_lam_init, dim3(nblocks), dim3(256), 0, s, m, ghist, lam);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_counter(int *ctr, int which, int value, int add, hipStream_t s) {
+  hipLaunchKernelGGL(kb_counter, dim3(1), dim3(64), 0, s, ctr, which, value, add);
   return hipGetLastError();
 }
 

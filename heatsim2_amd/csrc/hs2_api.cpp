@@ -13,6 +13,7 @@
 
 #include "../../include/heatsim2_amd.h"
 #include "hs2_adjoint.h"
+#include "hs2_batch.h"
 #include "hs2_launch.h"
 
 namespace {
@@ -62,6 +63,26 @@ struct hs2_plan {
   int *d_check = nullptr;       // per-block maxima of the maps (validation)
   hs2::XConfig xcfg{0, 0, 0};
   hs2::SConfig scfg[2] = {{0, 0}, {0, 0}};  // [0] z, [1] y
+};
+
+enum { KIND_X2 = 0, KIND_Y = 1, KIND_Z = 2, KIND_AX = 3, KIND_CELL = 4, NKIND = 5 };
+
+struct hs2_batch {
+  int ninst = 0, K = 0, maxsteps = 0, device = 0;
+  std::vector<hs2::BatchInst> inst;
+  long long N = 0, H = 0, CK = 0;  // cells, history values, checkpoint values
+  hs2::BatchInst *d_inst = nullptr;
+  int *d_ctr = nullptr;
+  int nblk[NKIND] = {};
+  int *d_blk_inst[NKIND] = {};
+  int *d_blk0[NKIND] = {};
+  hs2::DenseView dv{};
+  bool have_coef = false;
+  int use_graph = 1, chunk = 8;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t fwd_exec = nullptr, bwd_exec = nullptr;
+  std::vector<const void *> fwd_key, bwd_key;
 };
 
 namespace {
@@ -227,6 +248,171 @@ void free_coefficients(hs2_plan *p) {
   p->d_dense = nullptr;
   p->d_densew = nullptr;
   p->mode = MODE_NONE;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- batches
+namespace {
+
+hs2::BatchMap batch_map(const hs2_batch *b, int kind, int rule, int j) {
+  return hs2::BatchMap{b->d_inst, b->d_blk_inst[kind], b->d_blk0[kind], b->d_ctr, rule, j, b->K};
+}
+
+int blocks_of(const hs2::BatchInst &I, int kind) {
+  const int nbx = (I.nx + hs2::kBatchCW - 1) / hs2::kBatchCW;
+  switch (kind) {
+    case KIND_X2: return (I.nz * I.ny + hs2::kBatchXLines - 1) / hs2::kBatchXLines;
+    case KIND_Y: return nbx * I.nz;
+    case KIND_Z: return nbx * I.ny;
+    case KIND_AX: return (I.nz * I.ny + hs2::kBatchCW - 1) / hs2::kBatchCW;
+    default: return (int)(((long long)I.nz * I.ny * I.nx + 255) / 256);
+  }
+}
+
+void batch_free(hs2_batch *b) {
+  if (b->fwd_exec) (void)hipGraphExecDestroy(b->fwd_exec);
+  if (b->bwd_exec) (void)hipGraphExecDestroy(b->bwd_exec);
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  (void)hipFree(b->d_inst);
+  (void)hipFree(b->d_ctr);
+  for (int k = 0; k < NKIND; k++) {
+    (void)hipFree(b->d_blk_inst[k]);
+    (void)hipFree(b->d_blk0[k]);
+  }
+}
+
+// work layout: ws (N) | checkpoints (CK) | segment (K N) | Tx Ty Tn1 ux uy uz lam0 lam1 (8 N)
+struct BatchWork {
+  double *ws, *ck, *seg, *Tx, *Ty, *Tn1, *ux, *uy, *uz, *lam[2];
+};
+BatchWork batch_work(const hs2_batch *b, double *w) {
+  BatchWork r;
+  r.ws = w;
+  r.ck = r.ws + b->N;
+  r.seg = r.ck + b->CK;
+  r.Tx = r.seg + (size_t)b->K * b->N;
+  r.Ty = r.Tx + b->N;
+  r.Tn1 = r.Ty + b->N;
+  r.ux = r.Tn1 + b->N;
+  r.uy = r.ux + b->N;
+  r.uz = r.uy + b->N;
+  r.lam[0] = r.uz + b->N;
+  r.lam[1] = r.lam[0] + b->N;
+  return r;
+}
+
+// forward: `steps` steps (ckpt store, x, y, z with the readout, tick)
+hipError_t emit_forward(hs2_batch *b, double *T, const double *src, double *hist,
+                        const BatchWork &w, int steps, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  for (int c = 0; c < steps && e == hipSuccess; c++) {
+    e = hs2::launch_batch_ckpt_store(batch_map(b, KIND_CELL, hs2::RULE_FWD, 0), b->nblk[KIND_CELL],
+                                     T, w.ck, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_x2(batch_map(b, KIND_X2, hs2::RULE_FWD, 0), b->nblk[KIND_X2], b->dv, T,
+                               w.ws, src, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_strided(batch_map(b, KIND_Y, hs2::RULE_FWD, 0), b->nblk[KIND_Y], 1,
+                                    b->dv, T, w.ws, w.ws, nullptr, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_strided(batch_map(b, KIND_Z, hs2::RULE_FWD, 0), b->nblk[KIND_Z], 0,
+                                    b->dv, T, w.ws, T, hist, s);
+    if (e == hipSuccess) e = hs2::launch_batch_counter(b->d_ctr, 0, 1, 1, s);
+  }
+  return e;
+}
+
+// one backward segment: load checkpoints, recompute K-1 steps, K adjoint steps, tick
+hipError_t emit_backward_segment(hs2_batch *b, const double *src, const double *ghist,
+                                 double *gM, double *gz, double *gy, double *gx, double *gsrc,
+                                 const BatchWork &w, hipStream_t s) {
+  const int K = b->K;
+  const size_t N = (size_t)b->N;
+  hipError_t e = hs2::launch_batch_ckpt_load(batch_map(b, KIND_CELL, hs2::RULE_SEG, 0),
+                                             b->nblk[KIND_CELL], w.ck, w.seg, b->N, s);
+  for (int j = 0; j + 1 < K && e == hipSuccess; j++) {
+    double *in = w.seg + (size_t)j * N, *out = w.seg + (size_t)(j + 1) * N;
+    e = hs2::launch_batch_x2(batch_map(b, KIND_X2, hs2::RULE_RECOMP, j), b->nblk[KIND_X2], b->dv,
+                             in, w.Tx, src, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_strided(batch_map(b, KIND_Y, hs2::RULE_RECOMP, j), b->nblk[KIND_Y], 1,
+                                    b->dv, in, w.Tx, w.Ty, nullptr, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_strided(batch_map(b, KIND_Z, hs2::RULE_RECOMP, j), b->nblk[KIND_Z], 0,
+                                    b->dv, in, w.Ty, out, nullptr, s);
+  }
+  for (int j = 0; j < K && e == hipSuccess; j++) {
+    const double *Tn = w.seg + (size_t)(K - 1 - j) * N;
+    e = hs2::launch_batch_x2(batch_map(b, KIND_X2, hs2::RULE_BWD, j), b->nblk[KIND_X2], b->dv, Tn,
+                             w.Tx, src, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_strided(batch_map(b, KIND_Y, hs2::RULE_BWD, j), b->nblk[KIND_Y], 1,
+                                    b->dv, Tn, w.Tx, w.Ty, nullptr, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_strided(batch_map(b, KIND_Z, hs2::RULE_BWD, j), b->nblk[KIND_Z], 0,
+                                    b->dv, Tn, w.Ty, w.Tn1, nullptr, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_adj_line(batch_map(b, KIND_Z, hs2::RULE_BWD, j), b->nblk[KIND_Z], 0,
+                                     b->dv, w.lam[j & 1], 0, w.uz, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_adj_line(batch_map(b, KIND_Y, hs2::RULE_BWD, j), b->nblk[KIND_Y], 1,
+                                     b->dv, w.uz, 1, w.uy, s);
+    if (e == hipSuccess)
+      e = hs2::launch_batch_adj_line(batch_map(b, KIND_AX, hs2::RULE_BWD, j), b->nblk[KIND_AX], 2,
+                                     b->dv, w.uy, 1, w.ux, s);
+    if (e == hipSuccess) {
+      hs2::AdjFields f{};
+      f.Tn = Tn; f.Tx = w.Tx; f.Ty = w.Ty; f.Tn1 = w.Tn1;
+      f.ux = w.ux; f.uy = w.uy; f.uz = w.uz;
+      f.ghist = ghist;
+      f.lam_out = w.lam[(j + 1) & 1];
+      f.gM = gM;
+      f.gg[0] = gz; f.gg[1] = gy; f.gg[2] = gx;
+      f.gsrc = gsrc;
+      e = hs2::launch_batch_adj_stencil(batch_map(b, KIND_CELL, hs2::RULE_BWD, j),
+                                        b->nblk[KIND_CELL], b->dv, f, s);
+    }
+  }
+  if (e == hipSuccess) e = hs2::launch_batch_counter(b->d_ctr, 1, 1, 1, s);
+  return e;
+}
+
+// run `emit` `reps` times on b->stream, through a graph captured once per key
+template <class Emit>
+int batch_replay(hs2_batch *b, hipGraphExec_t *exec, std::vector<const void *> *key,
+                 const std::vector<const void *> &want, int reps, Emit emit, const char *who) {
+  hipStream_t s = b->stream;
+  if (!b->use_graph) {
+    for (int r = 0; r < reps; r++) HIP_TRY(emit(s));
+    return HS2_OK;
+  }
+  if (!*exec || *key != want) {
+    if (*exec) {
+      HIP_TRY(hipGraphExecDestroy(*exec));
+      *exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const hipError_t ce = emit(s);
+    const hipError_t ee = hipStreamEndCapture(s, &graph);
+    if (ce != hipSuccess || ee != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(HS2_E_HIP, "%s: graph capture failed: %s", who,
+                  hipGetErrorString(ce != hipSuccess ? ce : ee));
+    }
+    const hipError_t ie = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      *exec = nullptr;
+      return fail(HS2_E_HIP, "%s: hipGraphInstantiate: %s", who, hipGetErrorString(ie));
+    }
+    *key = want;
+  }
+  for (int r = 0; r < reps; r++) HIP_TRY(hipGraphLaunch(*exec, s));
+  return HS2_OK;
 }
 
 }  // namespace
@@ -686,6 +872,190 @@ int hs2_grad_backward(hs2_plan *p, int nsteps, int ckpt_every, const double *d_c
   }
   if (d_gT0)
     HIP_TRY(hipMemcpyAsync(d_gT0, lam, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  return HS2_OK;
+}
+
+int hs2_batch_create(const hs2_batch_inst *in, int ninst, int ckpt_every, hs2_batch **out) {
+  if (!in || !out || ninst < 1 || ckpt_every < 2)
+    return fail(HS2_E_ARG, "hs2_batch_create: bad argument");
+  auto *b = new (std::nothrow) hs2_batch();
+  if (!b) return fail(HS2_E_ALLOC, "hs2_batch_create: out of host memory");
+  b->ninst = ninst;
+  b->K = (ckpt_every + 1) & ~1;  // even: every segment starts on the same adjoint buffer
+  long long off = 0, hoff = 0, ck = 0;
+  for (int i = 0; i < ninst; i++) {
+    const hs2_batch_inst &s = in[i];
+    if (s.nz < 2 || s.ny < 1 || s.nx < 1 || s.nz > hs2::kBatchMaxLen ||
+        s.ny > hs2::kBatchMaxLen || s.nx > hs2::kBatchMaxLen || s.nsteps < 1 ||
+        s.src_step >= s.nsteps || !(s.dz > 0)) {
+      delete b;
+      return fail(HS2_E_ARG, "hs2_batch_create: instance %d: grid %dx%dx%d, nsteps %d, source "
+                  "step %d (axes 1..%d, nz >= 2)", i, s.nz, s.ny, s.nx, s.nsteps, s.src_step,
+                  hs2::kBatchMaxLen);
+    }
+    hs2::BatchInst I{};
+    I.nz = s.nz; I.ny = s.ny; I.nx = s.nx; I.nsteps = s.nsteps;
+    I.src_step = s.src_step; I.dz = s.dz;
+    const long long n = (long long)s.nz * s.ny * s.nx;
+    I.off = off;
+    I.hoff = hoff;
+    I.ckoff = ck;
+    off += (n + hs2::kBatchAlign - 1) / hs2::kBatchAlign * hs2::kBatchAlign;
+    hoff += (long long)s.nsteps * s.ny * s.nx;
+    ck += (long long)((s.nsteps + b->K - 1) / b->K) * n;
+    b->maxsteps = std::max(b->maxsteps, s.nsteps);
+    b->inst.push_back(I);
+  }
+  b->N = off;
+  b->H = hoff;
+  b->CK = ck;
+  int rc = HS2_OK;
+  auto bad = [&](hipError_t e, const char *what) {
+    if (e != hipSuccess && rc == HS2_OK)
+      rc = fail(HS2_E_HIP, "hs2_batch_create: %s: %s", what, hipGetErrorString(e));
+  };
+  bad(hipGetDevice(&b->device), "hipGetDevice");
+  bad(hipMalloc(&b->d_inst, sizeof(hs2::BatchInst) * ninst), "hipMalloc");
+  bad(hipMalloc(&b->d_ctr, 2 * sizeof(int)), "hipMalloc");
+  if (rc == HS2_OK)
+    bad(hipMemcpy(b->d_inst, b->inst.data(), sizeof(hs2::BatchInst) * ninst,
+                  hipMemcpyHostToDevice), "hipMemcpy");
+  for (int k = 0; k < NKIND && rc == HS2_OK; k++) {
+    std::vector<int> b0(ninst + 1, 0), owner;
+    for (int i = 0; i < ninst; i++) {
+      const int nb = blocks_of(b->inst[i], k);
+      b0[i + 1] = b0[i] + nb;
+      owner.insert(owner.end(), nb, i);
+    }
+    b->nblk[k] = b0[ninst];
+    bad(hipMalloc(&b->d_blk0[k], sizeof(int) * (ninst + 1)), "hipMalloc");
+    bad(hipMalloc(&b->d_blk_inst[k], sizeof(int) * owner.size()), "hipMalloc");
+    if (rc == HS2_OK) {
+      bad(hipMemcpy(b->d_blk0[k], b0.data(), sizeof(int) * (ninst + 1), hipMemcpyHostToDevice),
+          "hipMemcpy");
+      bad(hipMemcpy(b->d_blk_inst[k], owner.data(), sizeof(int) * owner.size(),
+                    hipMemcpyHostToDevice), "hipMemcpy");
+    }
+  }
+  if (rc == HS2_OK) bad(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+  if (rc == HS2_OK) bad(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming), "event");
+  if (rc == HS2_OK) bad(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming), "event");
+  if (rc != HS2_OK) {
+    batch_free(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return HS2_OK;
+}
+
+void hs2_batch_destroy(hs2_batch *b) {
+  if (!b) return;
+  (void)hipDeviceSynchronize();
+  batch_free(b);
+  delete b;
+}
+
+int hs2_batch_layout(const hs2_batch *b, int64_t *cell_off, int64_t *hist_off, int64_t *sizes) {
+  if (!b) return fail(HS2_E_ARG, "hs2_batch_layout: null batch");
+  for (int i = 0; i < b->ninst; i++) {
+    if (cell_off) cell_off[i] = b->inst[i].off;
+    if (hist_off) hist_off[i] = b->inst[i].hoff;
+  }
+  if (cell_off) cell_off[b->ninst] = b->N;
+  if (hist_off) hist_off[b->ninst] = b->H;
+  if (sizes) {
+    sizes[0] = b->N;
+    sizes[1] = b->H;
+    sizes[2] = (int64_t)b->N * (b->K + 9) + b->CK;  // work doubles
+    sizes[3] = b->K;
+    sizes[4] = b->maxsteps;
+  }
+  return HS2_OK;
+}
+
+int hs2_batch_set_dense(hs2_batch *b, const double *d_M, const uint8_t *d_w, const double *d_gz,
+                        const double *d_gy, const double *d_gx) {
+  if (!b || !d_M || !d_w || !d_gz || !d_gy || !d_gx)
+    return fail(HS2_E_ARG, "hs2_batch_set_dense: null argument");
+  if (!aligned16(d_M) || !aligned16(d_w) || !aligned16(d_gz) || !aligned16(d_gy) ||
+      !aligned16(d_gx))
+    return fail(HS2_E_ARG, "hs2_batch_set_dense: arrays must be 16-byte aligned");
+  b->dv.M = d_M;
+  b->dv.w = d_w;
+  b->dv.g[0] = d_gz;
+  b->dv.g[1] = d_gy;
+  b->dv.g[2] = d_gx;
+  b->have_coef = true;
+  return HS2_OK;
+}
+
+int hs2_batch_set_options(hs2_batch *b, int use_graph, int chunk) {
+  if (!b || chunk < 1) return fail(HS2_E_ARG, "hs2_batch_set_options: bad argument");
+  b->use_graph = use_graph ? 1 : 0;
+  b->chunk = chunk;
+  return HS2_OK;
+}
+
+int hs2_batch_forward(hs2_batch *b, double *d_T, const double *d_src, double *d_hist,
+                      double *d_work, size_t work_doubles, void *stream) {
+  if (!b || !d_T || !d_hist || !d_work) return fail(HS2_E_ARG, "hs2_batch_forward: null argument");
+  if (!b->have_coef) return fail(HS2_E_STATE, "hs2_batch_forward: coefficients not set");
+  if (!aligned16(d_T) || !aligned16(d_work) || (d_src && !aligned16(d_src)))
+    return fail(HS2_E_ARG, "hs2_batch_forward: fields must be 16-byte aligned");
+  if (work_doubles < (size_t)(b->N * (b->K + 9) + b->CK))
+    return fail(HS2_E_ARG, "hs2_batch_forward: work buffer too small");
+  const BatchWork w = batch_work(b, d_work);
+  HIP_TRY(hipEventRecord(b->ev_in, (hipStream_t)stream));
+  HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+  HIP_TRY(hs2::launch_batch_counter(b->d_ctr, 0, 0, 0, b->stream));
+  const int C = std::min(b->chunk, b->maxsteps);
+  const int reps = (b->maxsteps + C - 1) / C;
+  const std::vector<const void *> key = {d_T, d_src, d_hist, d_work, b->dv.M, b->dv.w,
+                                         b->dv.g[0], b->dv.g[1], b->dv.g[2],
+                                         (const void *)(intptr_t)C};
+  const int rc = batch_replay(b, &b->fwd_exec, &b->fwd_key, key, reps,
+                              [&](hipStream_t s) { return emit_forward(b, d_T, d_src, d_hist, w, C, s); },
+                              "hs2_batch_forward");
+  if (rc != HS2_OK) return rc;
+  HIP_TRY(hipEventRecord(b->ev_out, b->stream));
+  HIP_TRY(hipStreamWaitEvent((hipStream_t)stream, b->ev_out, 0));
+  return HS2_OK;
+}
+
+int hs2_batch_backward(hs2_batch *b, const double *d_src, const double *d_grad_hist,
+                       double *d_gM, double *d_gz, double *d_gy, double *d_gx, double *d_gsrc,
+                       double *d_work, size_t work_doubles, void *stream) {
+  if (!b || !d_grad_hist || !d_gM || !d_gz || !d_gy || !d_gx || !d_work)
+    return fail(HS2_E_ARG, "hs2_batch_backward: null argument");
+  if (!b->have_coef) return fail(HS2_E_STATE, "hs2_batch_backward: coefficients not set");
+  if (work_doubles < (size_t)(b->N * (b->K + 9) + b->CK))
+    return fail(HS2_E_ARG, "hs2_batch_backward: work buffer too small");
+  const BatchWork w = batch_work(b, d_work);
+  hipStream_t s = b->stream;
+  HIP_TRY(hipEventRecord(b->ev_in, (hipStream_t)stream));
+  HIP_TRY(hipStreamWaitEvent(s, b->ev_in, 0));
+  HIP_TRY(hs2::launch_fill(d_gM, b->N, 0.0, s));
+  HIP_TRY(hs2::launch_fill(d_gz, b->N, 0.0, s));
+  HIP_TRY(hs2::launch_fill(d_gy, b->N, 0.0, s));
+  HIP_TRY(hs2::launch_fill(d_gx, b->N, 0.0, s));
+  if (d_gsrc) HIP_TRY(hs2::launch_fill(d_gsrc, b->N, 0.0, s));
+  HIP_TRY(hs2::launch_batch_lam_init(batch_map(b, KIND_CELL, hs2::RULE_ALL, 0), b->nblk[KIND_CELL],
+                                     d_grad_hist, w.lam[0], s));
+  HIP_TRY(hs2::launch_batch_counter(b->d_ctr, 1, 0, 0, s));
+  const int reps = (b->maxsteps + b->K - 1) / b->K;
+  const std::vector<const void *> key = {d_src, d_grad_hist, d_gM, d_gz, d_gy, d_gx, d_gsrc,
+                                         d_work, b->dv.M, b->dv.w, b->dv.g[0], b->dv.g[1],
+                                         b->dv.g[2]};
+  const int rc = batch_replay(
+      b, &b->bwd_exec, &b->bwd_key, key, reps,
+      [&](hipStream_t st) {
+        return emit_backward_segment(b, d_src, d_grad_hist, d_gM, d_gz, d_gy, d_gx, d_gsrc, w, st);
+      },
+      "hs2_batch_backward");
+  if (rc != HS2_OK) return rc;
+  HIP_TRY(hipEventRecord(b->ev_out, s));
+  HIP_TRY(hipStreamWaitEvent((hipStream_t)stream, b->ev_out, 0));
   return HS2_OK;
 }
 

@@ -1,0 +1,107 @@
+// Batched execution of many independent ADI problems of different sizes and
+// step counts: one launch per sweep covers every instance.  Each launch kind
+// has a block -> instance table; a block finds its instance, its local block
+// index and its instance's current step, and works on that instance's slice
+// of the concatenated fields (per-voxel coefficients, DenseView).
+//
+// Time is driven by counters in device memory (ctr[0]: forward step,
+// ctr[1]: backward segment), so a captured graph of a chunk of steps can be
+// replayed without changing any kernel argument.  Checkpoints are aligned
+// to each instance's end: with K steps per segment, backward segment s of
+// instance i covers steps [max(0, e-K), e), e = nsteps_i - s*K, so every
+// instance is at the same position inside its segment and the recomputation
+// runs in lockstep.  Segment slot q holds T^{e-K+q}.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hs2_device.h"
+
+namespace hs2 {
+
+constexpr int kBatchMaxLen = 128;  // axis length limit of the batched kernels
+constexpr int kBatchCW = 16;       // lines per block of the strided / line kernels
+constexpr int kBatchSeg = 8;       // cells per thread segment (16 segments x 8 = 128)
+constexpr int kBatchXLines = 8;    // x-lines per block of the batched x sweep
+constexpr int kBatchAlign = 32;    // instance offsets are multiples of this many cells
+
+struct BatchInst {
+  int nz, ny, nx, nsteps;
+  long long off;    // first cell in the concatenated fields
+  long long hoff;   // first value of the surface history (nsteps x ny x nx)
+  long long ckoff;  // first value of the checkpoints (ceil(nsteps/K) fields)
+  int src_step;     // step whose source field applies (-1: none)
+  int pad;
+  double dz;
+};
+
+enum BatchRule { RULE_ALL = 0, RULE_FWD = 1, RULE_RECOMP = 2, RULE_BWD = 3, RULE_SEG = 4 };
+
+struct BatchMap {
+  const BatchInst *inst;
+  const int *blk_inst;  // instance of each block of this launch kind
+  const int *blk0;      // first block of each instance
+  const int *ctr;       // device counters
+  int rule, j, K;
+};
+
+// Instance, local block and step of this block; false if the instance has
+// nothing to do under the launch's rule (the whole block returns).
+__device__ __forceinline__ bool batch_locate(const BatchMap &m, BatchInst &I, int &local, int &t) {
+  const int gb = blockIdx.x;
+  const int i = m.blk_inst[gb];
+  local = gb - m.blk0[i];
+  I = m.inst[i];
+  if (m.rule == RULE_ALL) {
+    t = 0;
+    return true;
+  }
+  if (m.rule == RULE_FWD) {
+    t = m.ctr[0];
+    return t < I.nsteps;
+  }
+  const int e = I.nsteps - m.ctr[1] * m.K;
+  if (e <= 0) return false;
+  const int s = e - m.K > 0 ? e - m.K : 0;
+  if (m.rule == RULE_SEG) {
+    t = s;
+    return true;
+  }
+  if (m.rule == RULE_RECOMP) {
+    t = e - m.K + m.j;
+    return t >= s && t < e - 1;
+  }
+  t = e - 1 - m.j;  // RULE_BWD
+  return t >= s;
+}
+
+__device__ __forceinline__ DenseView offset_view(DenseView v, long long off) {
+  v.M += off;
+  v.w += off;
+  v.g[0] += off;
+  v.g[1] += off;
+  v.g[2] += off;
+  return v;
+}
+
+}  // namespace hs2
+
+namespace hs2 {
+struct AdjFields;
+// launchers (hs2_kernels.hip, hs2_adjoint.hip); nblocks = this kind's block total
+hipError_t launch_batch_x2(const BatchMap &m, int nblocks, const DenseView &v, const double *Tn,
+                           double *Ts, const double *src, hipStream_t s);
+hipError_t launch_batch_strided(const BatchMap &m, int nblocks, int axis, const DenseView &v,
+                                const double *Tn, const double *Y, double *X, double *hist,
+                                hipStream_t s);
+hipError_t launch_batch_adj_line(const BatchMap &m, int nblocks, int axis, const DenseView &v,
+                                 const double *R, int scale_m, double *U, hipStream_t s);
+hipError_t launch_batch_adj_stencil(const BatchMap &m, int nblocks, const DenseView &v,
+                                    const AdjFields &f, hipStream_t s);
+hipError_t launch_batch_ckpt_store(const BatchMap &m, int nblocks, const double *T, double *ck,
+                                   hipStream_t s);
+hipError_t launch_batch_ckpt_load(const BatchMap &m, int nblocks, const double *ck, double *seg,
+                                  long long N, hipStream_t s);
+hipError_t launch_batch_lam_init(const BatchMap &m, int nblocks, const double *ghist, double *lam,
+                                 hipStream_t s);
+hipError_t launch_batch_counter(int *ctr, int which, int value, int add, hipStream_t s);
+}  // namespace hs2

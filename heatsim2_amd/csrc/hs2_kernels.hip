@@ -22,6 +22,7 @@
 // Every global load goes through a clamped in-range address with no branch
 // around it, so a thread issues all of its loads before it first waits.
 #include "hs2_device.h"
+#include "hs2_batch.h"
 #include "hs2_launch.h"
 
 namespace hs2 {
@@ -131,12 +132,11 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
 // the class bytes and solves like k_sweep_x.  LPB lines per block.  SRC adds
 // the volumetric source terms (only on the rare steps that carry one).
 template <class V, int SEG, int G, int LPB, bool SRC>
-__global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
-                                                    const double *__restrict__ Tn,
-                                                    double *__restrict__ Ts,
-                                                    const uint8_t *__restrict__ vmap,
-                                                    const double *__restrict__ vtab_g,
-                                                    const double *__restrict__ sfield) {
+__device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__restrict__ Tn,
+                                              double *__restrict__ Ts,
+                                              const uint8_t *__restrict__ vmap,
+                                              const double *__restrict__ vtab_g,
+                                              const double *__restrict__ sfield, int blk) {
   constexpr int NT = G * LPB;
   constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;  // padded LDS row (doubles)
   constexpr int U = 4;                                // cells per thread per batch
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
     for (int t = threadIdx.x; t < kMaxVtab; t += NT) vtab[t] = vtab_g[t];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int nlines = nz * ny;
-  const int line0 = xcd_remap(blockIdx.x, gridDim.x) * LPB;
+  const int line0 = blk * LPB;
   const int tid = threadIdx.x;
   if (tid < LPB) {
     const int line = line0 + tid;
@@ -389,15 +389,26 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
   }
 }
 
+template <class V, int SEG, int G, int LPB, bool SRC>
+__global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
+                                                    const double *__restrict__ Tn,
+                                                    double *__restrict__ Ts,
+                                                    const uint8_t *__restrict__ vmap,
+                                                    const double *__restrict__ vtab_g,
+                                                    const double *__restrict__ sfield) {
+  sweep_x2_body<V, SEG, G, LPB, SRC>(g, view, Tn, Ts, vmap, vtab_g, sfield,
+                                     xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------- y / z sweeps
 // axis 1 (y): columns (k, i), line stride nx.  axis 0 (z): columns (j, i),
 // line stride nx*ny.  Solves (M - La/2) X = M Y - (La/2) T^n along each line.
 // X may alias Y or T^n: a block owns whole lines, and all of its loads
 // happen before its first barrier and all of its stores after its last.
 template <class V, int SEG, int AXIS>
-__global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, const double *Tn,
-                                                       const double *Y, double *X, double *surf,
-                                                       double dz) {
+__device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const double *Tn,
+                                                   const double *Y, double *X, double *surf,
+                                                   double dz, int bx, int by) {
   constexpr int axis = AXIS;
   extern __shared__ double lds[];
   view.load_tables(lds, axis);
@@ -409,10 +420,10 @@ __global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, c
 
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int cl = threadIdx.x % CW, p = threadIdx.x / CW;
-  const int i = blockIdx.x * CW + cl;
+  const int i = bx * CW + cl;
   const bool valid = i < nx;
   const int ic = valid ? i : nx - 1;
-  const int other = blockIdx.y;
+  const int other = by;
   const int L = axis == 1 ? ny : nz;
   const int stride = axis == 1 ? nx : nx * ny;
   const int base = axis == 1 ? other * ny * nx + ic : other * nx + ic;
@@ -492,6 +503,13 @@ __global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, c
     if (t < L) X[base + t * stride] = x[q];
   }
   if (surf && axis == 0 && p == 0 && L >= 2) surf[base] = surface_readout(x[0], x[1], dz);
+}
+
+template <class V, int SEG, int AXIS>
+__global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, const double *Tn,
+                                                       const double *Y, double *X, double *surf,
+                                                       double dz) {
+  sweep_strided_body<V, SEG, AXIS>(g, view, CW, Tn, Y, X, surf, dz, blockIdx.x, blockIdx.y);
 }
 
 // ------------------------------------------------------ K0: classify
@@ -704,6 +722,59 @@ hipError_t launch_surface(const double *T, int ny, int nx, double dz, double *ou
   int blocks = (sxy + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_surface, dim3(blocks), dim3(256), 0, s, T, ny, nx, dz, out);
+  return hipGetLastError();
+}
+
+
+// ------------------------------------------------------ batched sweeps
+// One launch covers every instance of a batch (per-voxel coefficients,
+// axis lengths <= kBatchMaxLen); see hs2_batch.h for the block mapping.
+template <int SEG, int G, int LPB>
+__global__ __launch_bounds__(G *LPB) void kb_sweep_x2(BatchMap m, DenseView v, const double *Tn,
+                                                     double *Ts, const double *src) {
+  BatchInst I;
+  int local, t;
+  if (!batch_locate(m, I, local, t)) return;
+  const double *sf = (src && t == I.src_step) ? src + I.off : nullptr;
+  sweep_x2_body<DenseView, SEG, G, LPB, true>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off),
+                                              Tn + I.off, Ts + I.off, nullptr, nullptr, sf,
+                                              local);
+}
+
+template <int SEG, int AXIS>
+__global__ __launch_bounds__(kBatchCW *kBatchMaxLen / kBatchSeg) void kb_sweep_strided(
+    BatchMap m, DenseView v, const double *Tn, const double *Y, double *X, double *hist) {
+  BatchInst I;
+  int local, t;
+  if (!batch_locate(m, I, local, t)) return;
+  const int nbx = (I.nx + kBatchCW - 1) / kBatchCW;
+  double *surf = (AXIS == 0 && hist) ? hist + I.hoff + (long long)t * I.ny * I.nx : nullptr;
+  sweep_strided_body<DenseView, SEG, AXIS>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off),
+                                           kBatchCW, Tn + I.off, Y + I.off, X + I.off, surf, I.dz,
+                                           local % nbx, local / nbx);
+}
+
+hipError_t launch_batch_x2(const BatchMap &m, int nblocks, const DenseView &v, const double *Tn,
+                           double *Ts, const double *src, hipStream_t s) {
+  constexpr int SEG = kBatchMaxLen / 16, G = 16, LPB = kBatchXLines;
+  constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;
+  const size_t shm = sizeof(double) * (kMaxVtab + LPB * RS);
+  hipLaunchKernelGGL((kb_sweep_x2<SEG, G, LPB>), dim3(nblocks), dim3(G * LPB), shm, s, m, v, Tn,
+                     Ts, src);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_strided(const BatchMap &m, int nblocks, int axis, const DenseView &v,
+                                const double *Tn, const double *Y, double *X, double *hist,
+                                hipStream_t s) {
+  constexpr int P = kBatchMaxLen / kBatchSeg;
+  const size_t shm = sizeof(double) * 6 * (size_t)P * kBatchCW;
+  if (axis == 1)
+    hipLaunchKernelGGL((kb_sweep_strided<kBatchSeg, 1>), dim3(nblocks), dim3(P * kBatchCW), shm, s,
+                       m, v, Tn, Y, X, hist);
+  else
+    hipLaunchKernelGGL((kb_sweep_strided<kBatchSeg, 0>), dim3(nblocks), dim3(P * kBatchCW), shm, s,
+                       m, v, Tn, Y, X, hist);
   return hipGetLastError();
 }
 

@@ -169,6 +169,56 @@ int hs2_grad_backward(hs2_plan *plan, int nsteps, int ckpt_every, const double *
                       double *d_gx, double *d_gsrc, double *d_gT0, double *d_work,
                       size_t work_doubles, void *stream);
 
+/* ---- Batches (no upstream counterpart).  Many independent problems of
+ * different sizes and step counts, each with per-voxel coefficients, run
+ * forward and adjoint together: one launch per sweep covers every
+ * instance, and the time loop is a captured HIP graph (a chunk of forward
+ * steps, or one checkpoint segment of the backward pass) replayed on an
+ * internal stream ordered after `stream`.  Axis lengths 1..128 (nz >= 2).
+ * All fields are the instances' fields concatenated at the offsets that
+ * hs2_batch_layout reports (16-byte aligned; padding cells are ignored). */
+typedef struct hs2_batch hs2_batch;
+
+typedef struct {
+  int32_t nz, ny, nx;     /* grid                                              */
+  int32_t nsteps;         /* time steps of this instance                       */
+  int32_t src_step;       /* step whose source field applies, or -1            */
+  double dz;              /* z spacing (readout)                               */
+} hs2_batch_inst;
+
+/* ckpt_every: steps per checkpoint segment (rounded up to even). */
+int hs2_batch_create(const hs2_batch_inst *inst, int ninst, int ckpt_every, hs2_batch **out);
+void hs2_batch_destroy(hs2_batch *batch);
+
+/* cell_off[ninst+1], hist_off[ninst+1] (may be NULL): first cell of each
+ * instance in the concatenated fields and first value of its surface
+ * history (nsteps x ny x nx); the last entries are the totals.  sizes[5]
+ * (may be NULL): total cells, total history values, work doubles,
+ * checkpoint interval, largest step count. */
+int hs2_batch_layout(const hs2_batch *batch, int64_t *cell_off, int64_t *hist_off,
+                     int64_t *sizes);
+
+/* Concatenated per-voxel coefficients, as hs2_plan_set_dense. */
+int hs2_batch_set_dense(hs2_batch *batch, const double *d_M, const uint8_t *d_w,
+                        const double *d_gz, const double *d_gy, const double *d_gx);
+
+/* use_graph 0: launch every step from the host instead; chunk: forward
+ * steps per captured graph. */
+int hs2_batch_set_options(hs2_batch *batch, int use_graph, int chunk);
+
+/* Forward: d_T (concatenated T^0 in, final fields out), d_src (concatenated
+ * source fields, or NULL), d_hist (concatenated surface histories).  d_work
+ * keeps the checkpoints for hs2_batch_backward. */
+int hs2_batch_forward(hs2_batch *batch, double *d_T, const double *d_src, double *d_hist,
+                      double *d_work, size_t work_doubles, void *stream);
+
+/* Backward over the last forward (same coefficients, sources and work):
+ * writes dJ/dM, dJ/dg per axis and dJ/dS (d_gsrc may be NULL), all
+ * concatenated, from dJ/dhist. */
+int hs2_batch_backward(hs2_batch *batch, const double *d_src, const double *d_grad_hist,
+                       double *d_gM, double *d_gz, double *d_gy, double *d_gx, double *d_gsrc,
+                       double *d_work, size_t work_doubles, void *stream);
+
 /* Insulating-top surface readout of a (nz,ny,nx) field into (ny,nx):
  * T0 - 5/16 (T1 - T0).  Replaces insulating_z_min_surface_temperature
  * (surface_temperature.py:4-36). */

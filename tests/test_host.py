@@ -1,0 +1,175 @@
+"""Host-side logic that runs without a GPU: the C-ABI library's exports, the
+compact coefficient tables of setup(), setup()'s input validation, the grid
+builders, hs2_indexing, the launch-shape candidate lists and the rocprof
+summary reader."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import heatsim2_amd as hs
+from heatsim2_amd import _lib, autotune, crank_nicolson, hs2_indexing, rocprof_summary
+from hs2_cases import load_golden, to_package_args
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "heatsim2_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hs2_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    names = _header_functions()
+    assert len(names) >= 20
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == names
+    L = _lib.load()
+    assert L.hs2_version().decode().startswith("heatsim2_amd")
+
+
+def _compact_tables(case):
+    pa = to_package_args(hs, case)
+    a = pa["setup"]
+    m, bz, by, bx = a[13], a[14], a[15], a[16]
+    d = (a[3], a[4], a[5])
+    return crank_nicolson.build_tables(d, a[9], a[10], a[11], m, bz, by, bx), (m, bz, by, bx)
+
+
+@pytest.mark.parametrize("name", ["tiny_mixed", "tiny_aniso", "steel_cube40"])
+def test_compact_tables_match_oracle_assembly(name):
+    """Every COMPUTE cell's minus-face conductance and rho*c/dt, looked up in
+    setup()'s compact tables, equals the oracle's per-cell assembly (which
+    is pinned to the upstream's assembled operators)."""
+    case = load_golden(name)
+    (gtab, mtab, compute), (m, bz, by, bx) = _compact_tables(case)
+    ref = oracle.assemble_case(case).reshape(case["shape"] + (8,))
+    nz, ny, nx = case["shape"]
+    nmat = len(case["materials"])
+    comp = compute[m].astype(bool)
+    np.testing.assert_allclose(mtab[m][comp], ref[..., 0][comp], rtol=1e-15)
+    faces = ((0, bz[1:-1], (slice(1, None),)), (1, by[:, 1:-1], (slice(None), slice(1, None))),
+             (2, bx[:, :, 1:-1], (slice(None), slice(None), slice(1, None))))
+    for axis, bint, sl in faces:
+        lo = [slice(None)] * 3
+        lo[axis] = slice(0, -1)
+        hi = [slice(None)] * 3
+        hi[axis] = slice(1, None)
+        g = gtab[bint.astype(int), axis, m[tuple(lo)], m[tuple(hi)]]
+        # as seen by the plus-side cell (its minus face) where that cell computes
+        c_hi = comp[tuple(hi)]
+        np.testing.assert_allclose(g[c_hi], ref[tuple(hi)][..., 2 + 2 * axis][c_hi], rtol=1e-14,
+                                   atol=0)
+        # and by the minus-side cell (its plus face)
+        c_lo = comp[tuple(lo)]
+        np.testing.assert_allclose(g[c_lo], ref[tuple(lo)][..., 3 + 2 * axis][c_lo], rtol=1e-14,
+                                   atol=0)
+
+
+def _setup_args(name="tiny_mixed"):
+    return list(to_package_args(hs, load_golden(name))["setup"])
+
+
+def test_setup_rejects_bad_maps():
+    a = _setup_args()
+    a[13] = a[13].astype(np.int32)  # material map must be uint8
+    with pytest.raises(ValueError):
+        hs.setup(*a)
+    a = _setup_args()
+    a[14] = a[14][:-1]  # bz must be (nz+1, ny, nx)
+    with pytest.raises(ValueError):
+        hs.setup(*a)
+    a = _setup_args()
+    a[13] = a[13].copy()
+    a[13][0, 0, 0] = 9  # only 3 materials
+    with pytest.raises(IndexError):
+        hs.setup(*a)
+
+
+def test_setup_rejects_flux_through_outer_face():
+    a = _setup_args()
+    a[14] = a[14].copy()
+    a[14][0, 2, 2] = 0  # conducting class on the top outer face of a COMPUTE cell
+    with pytest.raises(ValueError, match="outer faces must be insulating"):
+        hs.setup(*a)
+
+
+def test_setup_rejects_wrong_conductivity_type():
+    a = _setup_args("tiny_aniso")
+    mats = list(a[10])
+    mats[0] = (hs.TEMPERATURE_COMPUTE, 5.0, mats[0][2], mats[0][3])  # scalar k, anisotropic law
+    a[10] = tuple(mats)
+    with pytest.raises((TypeError, ValueError)):
+        hs.setup(*a)
+
+
+def test_setup_modes_not_provided():
+    a = _setup_args()
+    with pytest.raises(NotImplementedError):
+        hs.setup(*a, top_surface_y_curvatures=np.zeros(3))
+    with pytest.raises(NotImplementedError):
+        hs.setup(*a, unaligned_anisotropic=True)
+
+
+def test_grid_builders_closed_forms():
+    g = hs.build_grid(0.0, 1.0, 4, -1.0, 1.0, 5, 0.0, 3.0, 6)
+    dz, dy, dx, z, y, x, zg, yg, xg, zb = g[:10]
+    assert (dz, dy, dx) == (0.25, 0.4, 0.5)
+    np.testing.assert_allclose(z, [0.125, 0.375, 0.625, 0.875])
+    np.testing.assert_allclose(y, -1.0 + 0.2 + 0.4 * np.arange(5), atol=1e-15)
+    assert zg.shape == (4, 5, 6) and np.all(zg[:, 0, 0] == z) and np.all(xg[0, 0, :] == x)
+    np.testing.assert_allclose(g[-2], np.sqrt(xg ** 2 + yg ** 2 + zg ** 2))
+    g2 = hs.build_grid_min_step(0.125, 0.25, 4, -0.8, 0.4, 5, 0.25, 0.5, 6)
+    np.testing.assert_allclose(g2[0], z)
+    g3 = hs.build_grid_min_step_edge(0.0, 0.25, 4, -1.0, 0.4, 5, 0.0, 0.5, 6)
+    np.testing.assert_allclose(g3[6], zb)
+    maps = hs.zero_elements(2, 3, 4)
+    assert [mm.shape for mm in maps] == [(2, 3, 4), (3, 3, 4), (2, 4, 4), (2, 3, 5), (2, 3, 4)]
+    assert all(mm.dtype == np.uint8 for mm in maps)
+
+
+def test_hs2_indexing_round_trip():
+    s = [0, 1, 1, 2]
+    shape = (4, 8, 16)
+    total = sum((shape[1] >> q) * (shape[2] >> q) for q in s)
+    seen = set()
+    for flat in range(total):
+        ci = hs2_indexing.compressed_index_from_single(flat, s, shape)
+        assert hs2_indexing.compressed_single_index(ci, s, shape) == flat
+        (k0, k1), (j0, j1), (i0, i1) = hs2_indexing.uncompressed_index_range(ci, s)
+        for j in range(j0, j1):
+            for i in range(i0, i1):
+                assert hs2_indexing.compressed_index((k0, j, i), s) == ci
+                seen.add((k0, j, i))
+    assert len(seen) == shape[0] * shape[1] * shape[2]
+    with pytest.raises(IndexError):
+        hs2_indexing.compressed_index_from_single(total, s, shape)
+
+
+def test_launch_shape_candidates_fit_the_kernels():
+    for nx in (1, 3, 16, 17, 100, 256, 300, 512, 1000, 1024):
+        cands = autotune.x_candidates(nx)
+        assert cands and all(c[0] * c[1] >= nx for c in cands)
+        assert all(c in autotune.X_COMBOS for c in cands)
+    for L in (1, 2, 5, 64, 100, 256, 517, 1024):
+        for seg, cw in autotune.strided_candidates(L):
+            assert -(-L // seg) * cw <= autotune.MAX_BLOCK
+
+
+def test_rocprof_summary_attribution_and_committed_traffic():
+    assert rocprof_summary.sweep_of("void hs2::k_sweep_strided<hs2::ClassView, 16, 1>(x)") == "y"
+    assert rocprof_summary.sweep_of("void hs2::k_sweep_strided<hs2::ClassView, 16, 0>(x)") == "z"
+    assert rocprof_summary.sweep_of("void hs2::k_sweep_x2<hs2::ClassView, 16, 16, 4, false>") == "x"
+    assert rocprof_summary.sweep_of("hs2::k_classify") is None
+    tr = rocprof_summary.read_traffic(os.path.join(ROOT, "profiles",
+                                                   "hbm_traffic_two_layer_256.txt"))
+    assert tr["cells"] == 256 ** 3
+    for k, algo in (("sweep_x", 16), ("sweep_y", 24), ("sweep_z", 24)):
+        assert algo * 256 ** 3 <= tr["traffic_bytes"][k] < 2 * algo * 256 ** 3
