@@ -70,6 +70,181 @@ def cpu_baseline(case, nsteps):
                        "assembly %.1f s untimed" % ((nsteps,) + tuple(case["shape"]) + (dt, t_setup)))
 
 
+# ---------------------------------------------------------------- batched
+BATCH_DT = 2e-3
+BATCH_D = (5e-4, 5e-4, 5e-4)
+BATCH_H = 100.0
+
+
+def batch_workload(ninst=256, seed=6, lo=32, hi=128, smin=100, smax=1000):
+    """Uneven instances: axes U{lo..hi}, steps U{smin..smax}, isotropic or
+    anisotropic (p = 1/2) k log-uniform on [0.1, 60] W/m/K per diagonal
+    component, rho*c log-uniform on [1e5, 5e6] J/m^3/K, 0-4 thin-layer
+    patches (h = 100 W/m^2/K) at random z-faces, flash E = 1e4 (1 + 0.2 U)
+    J/m^2 on the top layer at step 0; 0.5 mm cells, dt = 2 ms."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(ninst):
+        nz, ny, nx = (int(v) for v in rng.integers(lo, hi + 1, size=3))
+        nsteps = int(rng.integers(smin, smax + 1))
+        aniso = bool(rng.random() < 0.5)
+        lk = lambda: float(np.exp(rng.uniform(np.log(0.1), np.log(60.0))))
+        k = (lk(), lk(), lk()) if aniso else (lk(),) * 3
+        rhoc = float(np.exp(rng.uniform(np.log(1e5), np.log(5e6))))
+        bz = np.zeros((nz + 1, ny, nx), np.uint8)
+        by = np.zeros((nz, ny + 1, nx), np.uint8)
+        bx = np.zeros((nz, ny, nx + 1), np.uint8)
+        bz[0] = bz[-1] = 1
+        by[:, 0] = by[:, -1] = 1
+        bx[:, :, 0] = bx[:, :, -1] = 1
+        for _p in range(int(rng.integers(0, 5))):
+            kz = int(rng.integers(1, nz))
+            sy, sx = int(rng.integers(1, ny + 1)), int(rng.integers(1, nx + 1))
+            y0, x0 = int(rng.integers(0, ny - sy + 1)), int(rng.integers(0, nx - sx + 1))
+            bz[kz, y0:y0 + sy, x0:x0 + sx] = 2
+        E = 1e4 * (1 + 0.2 * rng.uniform(-1, 1))
+        out.append(dict(shape=(nz, ny, nx), nsteps=nsteps, k=k, rhoc=rhoc, aniso=aniso,
+                        maps=(bz, by, bx), E=E))
+    return out
+
+
+class _BatchProblem(object):
+    """Per-instance leaf parameters (homogeneous initial guess), sources and
+    observations of one batch workload."""
+
+    def __init__(self, hs, insts, dev):
+        import torch
+        self.hs, self.insts, self.dev = hs, insts, dev
+        self.params = []
+        for x in insts:
+            sh = x["shape"]
+            p = {n: torch.full(sh, v, dtype=torch.float64, device=dev, requires_grad=True)
+                 for n, v in (("kz", x["k"][0]), ("ky", x["k"][1]), ("kx", x["k"][2]),
+                              ("rhoc", x["rhoc"]))}
+            p["E"] = torch.tensor(x["E"], dtype=torch.float64, device=dev, requires_grad=True)
+            mask = torch.zeros(sh, dtype=torch.float64, device=dev)
+            mask[0] = 1.0
+            p["mask"] = mask
+            p["T0"] = torch.zeros(sh, dtype=torch.float64, device=dev)
+            self.params.append(p)
+        self.obs = None
+
+    def boundaries(self, x):
+        hs = self.hs
+        law = hs.boundary_conducting_anisotropic if x["aniso"] else hs.boundary_conducting
+        return ((law,), (hs.boundary_insulating,), (hs.boundary_thininsulatinglayer, BATCH_H))
+
+    def coef(self, i, scale=None):
+        x, p = self.insts[i], self.params[i]
+        f = (lambda t: t) if scale is None else scale
+        return self.hs.adjoint.per_voxel_coefficients(f(p["kz"]), f(p["ky"]), f(p["kx"]),
+                                                      f(p["rhoc"]), BATCH_DT, BATCH_D,
+                                                      self.boundaries(x), *x["maps"])
+
+    def src(self, i, factor=1.0):
+        p = self.params[i]
+        return factor * p["E"] * p["mask"] / (BATCH_D[0] * BATCH_DT)
+
+    def zero_grad(self):
+        for p in self.params:
+            for n in ("kz", "ky", "kx", "rhoc", "E"):
+                p[n].grad = None
+
+
+def batched_gradients(hs, ninst, seed, ckpt_every, plain_sample, log):
+    """Gradient evaluations per second (forward + loss + adjoint for every
+    instance) of the batched path and of the plain per-instance path."""
+    import torch
+    from heatsim2_amd import _lib
+    dev = _lib.device()
+    insts = batch_workload(ninst, seed)
+    prob = _BatchProblem(hs, insts, dev)
+    cells = sum(int(np.prod(x["shape"])) for x in insts)
+    cell_steps = sum(int(np.prod(x["shape"])) * x["nsteps"] for x in insts)
+    log("batch: %d instances, %.3g cells, %.3g cell-steps per evaluation"
+        % (ninst, cells, cell_steps))
+    gb = hs.batch.GradBatch([x["shape"] for x in insts], BATCH_D[0],
+                            [x["nsteps"] for x in insts], [0] * ninst, ckpt_every=ckpt_every)
+    T0s = [p["T0"] for p in prob.params]
+    # observations: the history of a perturbed "true" field plus 0.01 K noise
+    rng = np.random.default_rng(seed + 1)
+    with torch.no_grad():
+        truth = lambda t: t * 1.1
+        hist = gb.surface_history([prob.coef(i, truth) for i in range(ninst)], T0s,
+                                  [prob.src(i, 1.05) for i in range(ninst)])
+        prob.obs = [h + 0.01 * torch.tensor(rng.standard_normal(tuple(h.shape)), device=dev)
+                    for h in hist]
+
+    def batched_eval():
+        prob.zero_grad()
+        coefs = [prob.coef(i) for i in range(ninst)]
+        srcs = [prob.src(i) for i in range(ninst)]
+        hists = gb.surface_history(coefs, T0s, srcs)
+        loss = sum(((h - o) ** 2).sum() for h, o in zip(hists, prob.obs))
+        loss.backward()
+        return loss
+
+    batched_eval()  # captures the graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss_b = batched_eval().item()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    log("batched: %.3f s per evaluation, %.2f gradients/s" % (t_batch, ninst / t_batch))
+
+    # plain: the same instances one after another through the single-instance path
+    idx = list(range(ninst)) if plain_sample <= 0 else \
+        sorted(np.random.default_rng(seed + 2).choice(ninst, min(plain_sample, ninst),
+                                                      replace=False).tolist())
+
+    def plain_eval(i):
+        x, p = insts[i], prob.params[i]
+        gp = hs.adjoint.GradPlan(x["shape"], BATCH_D, BATCH_DT)
+        h = hs.adjoint.surface_history(gp, prob.coef(i), p["T0"], x["nsteps"],
+                                       sources=[(0, prob.src(i))], ckpt_every=ckpt_every)
+        loss = ((h - prob.obs[i]) ** 2).sum()
+        loss.backward()
+        return loss
+
+    gk = {i: {n: prob.params[i][n].grad.clone() for n in ("kz", "ky", "kx", "rhoc")}
+          for i in idx[:4]}
+    prob.zero_grad()
+    plain_eval(idx[0])  # warm-up (code objects, allocator)
+    prob.zero_grad()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss_p = 0.0
+    for i in idx:
+        loss_p += plain_eval(i).item()
+    torch.cuda.synchronize()
+    t_plain_sample = time.perf_counter() - t0
+    t_plain = t_plain_sample * ninst / len(idx)
+    log("plain: %.3f s for %d instances -> %.3f s per evaluation of all %d, %.2f gradients/s"
+        % (t_plain_sample, len(idx), t_plain, ninst, ninst / t_plain))
+    # the two paths agree (same instances, same loss terms)
+    # the conductivity gradients share one scale (same units; a lateral one is
+    # zero up to round-off when nothing breaks the lateral symmetry)
+    dev_rel = 0.0
+    for i, gi in gk.items():
+        kscale = max(float(gi[n].abs().max()) for n in ("kz", "ky", "kx"))
+        for n, g in gi.items():
+            scale = kscale if n != "rhoc" else float(g.abs().max())
+            gp = prob.params[i][n].grad
+            dev_rel = max(dev_rel, float((gp - g).abs().max()) / scale)
+    return dict(metric="batched fwd+adjoint gradients/s vs plain", instances=ninst,
+                gradients_per_s=ninst / t_batch, plain_gradients_per_s=ninst / t_plain,
+                speedup=t_plain / t_batch, seconds_batched=round(t_batch, 3),
+                seconds_plain=round(t_plain, 3),
+                plain_sample=("all" if len(idx) == ninst else
+                              "%d of %d instances, time scaled by %d/%d" % (len(idx), ninst,
+                                                                            ninst, len(idx))),
+                cells=cells, cell_steps=cell_steps, ckpt_every=ckpt_every,
+                loss=loss_b, max_rel_grad_diff_vs_plain=dev_rel,
+                workload="%d uneven instances (axes 32-128, 100-1000 steps, seed %d), per-voxel "
+                         "kz/ky/kx/rho*c and flash-amplitude gradients of an L2 surface-history "
+                         "misfit, FP64" % (ninst, seed))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +257,13 @@ def main():
     ap.add_argument("--config", default=None,
                     help="force launch shapes, e.g. 'x=4,64,0;y=16,16;z=16,16' (skips the tuner)")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-gradient measurement")
+    ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=256)
+    ap.add_argument("--batch-seed", type=int, default=6)
+    ap.add_argument("--ckpt-every", type=int, default=50)
+    ap.add_argument("--plain-sample", type=int, default=32,
+                    help="time the plain path on this many instances and scale (0: all)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256^3 float64 field 20 times (scales HBM counters)")
     args = ap.parse_args()
@@ -103,6 +285,12 @@ def main():
     import heatsim2_amd as hs
     from heatsim2_amd import _lib
     from hs2_cases import cases, to_package_args
+
+    if args.batch_only:
+        res = batched_gradients(hs, args.batch_instances, args.batch_seed, args.ckpt_every,
+                                args.plain_sample, log)
+        print(json.dumps(res), flush=True)
+        return
 
     nz, ny, nx = args.size
     case = cases.case_two_layer(nz=nz, ny=ny, nx=nx, nsteps=1000)
@@ -179,6 +367,15 @@ def main():
                                                        "y_lines", "z_seg", "z_lines", "mode")}},
            "roofline": roofline}
     del T, hist
+    P.release()
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        log("batched gradients ...")
+        out["batched"] = batched_gradients(hs, args.batch_instances, args.batch_seed,
+                                           args.ckpt_every, args.plain_sample, log)
+        torch.cuda.empty_cache()
+    else:
+        out["batched"] = None
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)

@@ -211,8 +211,8 @@ def surface_history(gplan, coef, T0, nsteps, sources=(), ckpt_every=50):
     M, w, gz, gy, gx of the grid's shape); T0: initial field; sources:
     sequence of (step, field) with field the volumetric power density
     (W/m^3) added during that step (at most one per step); ckpt_every:
-    steps between checkpoints (memory: ceil(nsteps/ckpt_every) + ckpt_every
-    + 8 fields).  Gradients flow to M, g, T0 and the source fields.
+    steps between checkpoints (memory: ceil(nsteps/ckpt_every) +
+    3*ckpt_every + 6 fields).  Gradients flow to M, g, T0 and the source fields.
     """
     nsteps, ckpt_every = int(nsteps), int(ckpt_every)
     if nsteps < 1 or ckpt_every < 1:

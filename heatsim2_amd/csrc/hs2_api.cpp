@@ -284,19 +284,20 @@ void batch_free(hs2_batch *b) {
   }
 }
 
-// work layout: ws (N) | checkpoints (CK) | segment (K N) | Tx Ty Tn1 ux uy uz lam0 lam1 (8 N)
+// work layout: ws (N) | checkpoints (CK) | segment T^t (K+1 N) | T* (K N) | T** (K N) |
+// ux uy uz lam0 lam1 (5 N)
 struct BatchWork {
-  double *ws, *ck, *seg, *Tx, *Ty, *Tn1, *ux, *uy, *uz, *lam[2];
+  double *ws, *ck, *seg, *segx, *segy, *ux, *uy, *uz, *lam[2];
 };
+long long batch_work_doubles(const hs2_batch *b) { return b->N * (3LL * b->K + 7) + b->CK; }
 BatchWork batch_work(const hs2_batch *b, double *w) {
   BatchWork r;
   r.ws = w;
   r.ck = r.ws + b->N;
   r.seg = r.ck + b->CK;
-  r.Tx = r.seg + (size_t)b->K * b->N;
-  r.Ty = r.Tx + b->N;
-  r.Tn1 = r.Ty + b->N;
-  r.ux = r.Tn1 + b->N;
+  r.segx = r.seg + (size_t)(b->K + 1) * b->N;
+  r.segy = r.segx + (size_t)b->K * b->N;
+  r.ux = r.segy + (size_t)b->K * b->N;
   r.uy = r.ux + b->N;
   r.uz = r.uy + b->N;
   r.lam[0] = r.uz + b->N;
@@ -325,7 +326,8 @@ hipError_t emit_forward(hs2_batch *b, double *T, const double *src, double *hist
   return e;
 }
 
-// one backward segment: load checkpoints, recompute K-1 steps, K adjoint steps, tick
+// one backward segment: load checkpoints, recompute its K steps keeping every
+// state, K adjoint steps, tick
 hipError_t emit_backward_segment(hs2_batch *b, const double *src, const double *ghist,
                                  double *gM, double *gz, double *gy, double *gx, double *gsrc,
                                  const BatchWork &w, hipStream_t s) {
@@ -333,30 +335,22 @@ hipError_t emit_backward_segment(hs2_batch *b, const double *src, const double *
   const size_t N = (size_t)b->N;
   hipError_t e = hs2::launch_batch_ckpt_load(batch_map(b, KIND_CELL, hs2::RULE_SEG, 0),
                                              b->nblk[KIND_CELL], w.ck, w.seg, b->N, s);
-  for (int j = 0; j + 1 < K && e == hipSuccess; j++) {
-    double *in = w.seg + (size_t)j * N, *out = w.seg + (size_t)(j + 1) * N;
+  for (int j = 0; j < K && e == hipSuccess; j++) {
+    const double *in = w.seg + (size_t)j * N;
+    double *tx = w.segx + (size_t)j * N, *ty = w.segy + (size_t)j * N;
     e = hs2::launch_batch_x2(batch_map(b, KIND_X2, hs2::RULE_RECOMP, j), b->nblk[KIND_X2], b->dv,
-                             in, w.Tx, src, s);
+                             in, tx, src, s);
     if (e == hipSuccess)
       e = hs2::launch_batch_strided(batch_map(b, KIND_Y, hs2::RULE_RECOMP, j), b->nblk[KIND_Y], 1,
-                                    b->dv, in, w.Tx, w.Ty, nullptr, s);
+                                    b->dv, in, tx, ty, nullptr, s);
     if (e == hipSuccess)
       e = hs2::launch_batch_strided(batch_map(b, KIND_Z, hs2::RULE_RECOMP, j), b->nblk[KIND_Z], 0,
-                                    b->dv, in, w.Ty, out, nullptr, s);
+                                    b->dv, in, ty, w.seg + (size_t)(j + 1) * N, nullptr, s);
   }
   for (int j = 0; j < K && e == hipSuccess; j++) {
-    const double *Tn = w.seg + (size_t)(K - 1 - j) * N;
-    e = hs2::launch_batch_x2(batch_map(b, KIND_X2, hs2::RULE_BWD, j), b->nblk[KIND_X2], b->dv, Tn,
-                             w.Tx, src, s);
-    if (e == hipSuccess)
-      e = hs2::launch_batch_strided(batch_map(b, KIND_Y, hs2::RULE_BWD, j), b->nblk[KIND_Y], 1,
-                                    b->dv, Tn, w.Tx, w.Ty, nullptr, s);
-    if (e == hipSuccess)
-      e = hs2::launch_batch_strided(batch_map(b, KIND_Z, hs2::RULE_BWD, j), b->nblk[KIND_Z], 0,
-                                    b->dv, Tn, w.Ty, w.Tn1, nullptr, s);
-    if (e == hipSuccess)
-      e = hs2::launch_batch_adj_line(batch_map(b, KIND_Z, hs2::RULE_BWD, j), b->nblk[KIND_Z], 0,
-                                     b->dv, w.lam[j & 1], 0, w.uz, s);
+    const size_t q = (size_t)(K - 1 - j);
+    e = hs2::launch_batch_adj_line(batch_map(b, KIND_Z, hs2::RULE_BWD, j), b->nblk[KIND_Z], 0,
+                                   b->dv, w.lam[j & 1], 0, w.uz, s);
     if (e == hipSuccess)
       e = hs2::launch_batch_adj_line(batch_map(b, KIND_Y, hs2::RULE_BWD, j), b->nblk[KIND_Y], 1,
                                      b->dv, w.uz, 1, w.uy, s);
@@ -365,7 +359,8 @@ hipError_t emit_backward_segment(hs2_batch *b, const double *src, const double *
                                      b->dv, w.uy, 1, w.ux, s);
     if (e == hipSuccess) {
       hs2::AdjFields f{};
-      f.Tn = Tn; f.Tx = w.Tx; f.Ty = w.Ty; f.Tn1 = w.Tn1;
+      f.Tn = w.seg + q * N; f.Tx = w.segx + q * N; f.Ty = w.segy + q * N;
+      f.Tn1 = w.seg + (q + 1) * N;
       f.ux = w.ux; f.uy = w.uy; f.uz = w.uz;
       f.ghist = ghist;
       f.lam_out = w.lam[(j + 1) & 1];
@@ -776,7 +771,7 @@ int hs2_run_timed(hs2_plan *p, double *d_T, int nsteps, double *kernel_ms, void 
 
 size_t hs2_grad_work_doubles(const hs2_plan *p, int ckpt_every) {
   if (!p || ckpt_every < 1) return 0;
-  return ((size_t)ckpt_every + 8) * (size_t)p->n;
+  return (3 * (size_t)ckpt_every + 6) * (size_t)p->n;
 }
 
 int hs2_grad_forward(hs2_plan *p, double *d_T, int nsteps, int ckpt_every, double *d_ckpt,
@@ -830,9 +825,11 @@ int hs2_grad_backward(hs2_plan *p, int nsteps, int ckpt_every, const double *d_c
   hipStream_t s = (hipStream_t)stream;
   const size_t n = (size_t)p->n, nyx = (size_t)g.ny * g.nx;
   const int K = ckpt_every;
-  double *seg = d_work;  // K fields: T^{n0} .. T^{n0+K-1}
-  double *Tx = seg + (size_t)K * n, *Ty = Tx + n, *Tn1 = Ty + n;
-  double *ux = Tn1 + n, *uy = ux + n, *uz = uy + n;
+  // segment store: T^{n0} .. T^{n0+K} (K+1 fields), and T*, T** of each of
+  // its K steps, so every state of the segment is computed once
+  double *seg = d_work;
+  double *segx = seg + (size_t)(K + 1) * n, *segy = segx + (size_t)K * n;
+  double *ux = segy + (size_t)K * n, *uy = ux + n, *uz = uy + n;
   double *lam = uz + n, *lam2 = lam + n;
   HIP_TRY(hs2::launch_fill(d_gM, n, 0.0, s));
   HIP_TRY(hs2::launch_fill(d_gz, n, 0.0, s));
@@ -844,22 +841,20 @@ int hs2_grad_backward(hs2_plan *p, int nsteps, int ckpt_every, const double *d_c
     const int n0 = sg * K, n1 = std::min(nsteps, n0 + K);
     HIP_TRY(hipMemcpyAsync(seg, d_ckpt + (size_t)sg * n, n * sizeof(double),
                            hipMemcpyDeviceToDevice, s));
-    for (int t = n0; t < n1 - 1; t++) {  // recompute the segment's states
+    for (int t = n0; t < n1; t++) {  // recompute the segment's states
+      const size_t q = (size_t)(t - n0);
       const double *sf = src[t] >= 0 ? d_src + (size_t)src[t] * n : nullptr;
-      rc = step_states(p, seg + (size_t)(t - n0) * n, Tx, Ty, seg + (size_t)(t - n0 + 1) * n, sf,
+      rc = step_states(p, seg + q * n, segx + q * n, segy + q * n, seg + (q + 1) * n, sf,
                        nullptr, s);
       if (rc != HS2_OK) return rc;
     }
     for (int t = n1 - 1; t >= n0; t--) {
-      const double *Tn = seg + (size_t)(t - n0) * n;
-      const double *sf = src[t] >= 0 ? d_src + (size_t)src[t] * n : nullptr;
-      rc = step_states(p, Tn, Tx, Ty, Tn1, sf, nullptr, s);
-      if (rc != HS2_OK) return rc;
+      const size_t q = (size_t)(t - n0);
       HIP_TRY(adj_line(p, 0, lam, 0, uz, s));
       HIP_TRY(adj_line(p, 1, uz, 1, uy, s));
       HIP_TRY(adj_line(p, 2, uy, 1, ux, s));
       hs2::AdjFields f{};
-      f.Tn = Tn; f.Tx = Tx; f.Ty = Ty; f.Tn1 = Tn1;
+      f.Tn = seg + q * n; f.Tx = segx + q * n; f.Ty = segy + q * n; f.Tn1 = seg + (q + 1) * n;
       f.ux = ux; f.uy = uy; f.uz = uz;
       f.ghist = t >= 1 ? d_grad_hist + (size_t)(t - 1) * nyx : nullptr;
       f.lam_out = lam2;
@@ -967,7 +962,7 @@ int hs2_batch_layout(const hs2_batch *b, int64_t *cell_off, int64_t *hist_off, i
   if (sizes) {
     sizes[0] = b->N;
     sizes[1] = b->H;
-    sizes[2] = (int64_t)b->N * (b->K + 9) + b->CK;  // work doubles
+    sizes[2] = batch_work_doubles(b);
     sizes[3] = b->K;
     sizes[4] = b->maxsteps;
   }
@@ -1003,7 +998,7 @@ int hs2_batch_forward(hs2_batch *b, double *d_T, const double *d_src, double *d_
   if (!b->have_coef) return fail(HS2_E_STATE, "hs2_batch_forward: coefficients not set");
   if (!aligned16(d_T) || !aligned16(d_work) || (d_src && !aligned16(d_src)))
     return fail(HS2_E_ARG, "hs2_batch_forward: fields must be 16-byte aligned");
-  if (work_doubles < (size_t)(b->N * (b->K + 9) + b->CK))
+  if (work_doubles < (size_t)batch_work_doubles(b))
     return fail(HS2_E_ARG, "hs2_batch_forward: work buffer too small");
   const BatchWork w = batch_work(b, d_work);
   HIP_TRY(hipEventRecord(b->ev_in, (hipStream_t)stream));
@@ -1029,7 +1024,7 @@ int hs2_batch_backward(hs2_batch *b, const double *d_src, const double *d_grad_h
   if (!b || !d_grad_hist || !d_gM || !d_gz || !d_gy || !d_gx || !d_work)
     return fail(HS2_E_ARG, "hs2_batch_backward: null argument");
   if (!b->have_coef) return fail(HS2_E_STATE, "hs2_batch_backward: coefficients not set");
-  if (work_doubles < (size_t)(b->N * (b->K + 9) + b->CK))
+  if (work_doubles < (size_t)batch_work_doubles(b))
     return fail(HS2_E_ARG, "hs2_batch_backward: work buffer too small");
   const BatchWork w = batch_work(b, d_work);
   hipStream_t s = b->stream;

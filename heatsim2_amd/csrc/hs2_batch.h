@@ -10,7 +10,8 @@
 // to each instance's end: with K steps per segment, backward segment s of
 // instance i covers steps [max(0, e-K), e), e = nsteps_i - s*K, so every
 // instance is at the same position inside its segment and the recomputation
-// runs in lockstep.  Segment slot q holds T^{e-K+q}.
+// runs in lockstep.  Segment slot q (0..K) holds T^{e-K+q}; T* and T** of
+// step e-K+q are kept in slot q of their own stores.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -66,9 +67,9 @@ __device__ __forceinline__ bool batch_locate(const BatchMap &m, BatchInst &I, in
     t = s;
     return true;
   }
-  if (m.rule == RULE_RECOMP) {
+  if (m.rule == RULE_RECOMP) {  // step t: T^t (slot j) -> T^{t+1} (slot j+1)
     t = e - m.K + m.j;
-    return t >= s && t < e - 1;
+    return t >= s;
   }
   t = e - 1 - m.j;  // RULE_BWD
   return t >= s;

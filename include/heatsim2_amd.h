@@ -146,7 +146,8 @@ int hs2_run_timed(hs2_plan *plan, double *d_T, int nsteps, double *kernel_ms, vo
  * ckpt_every steps.  Works in both coefficient modes; gradients with
  * respect to per-voxel parameters need hs2_plan_set_dense. */
 
-/* Doubles of device workspace hs2_grad_backward needs: (ckpt_every+8)*n. */
+/* Doubles of device workspace hs2_grad_backward needs: (3*ckpt_every+6)*n
+ * (each segment's states T^n, T*, T** are recomputed once and kept). */
 size_t hs2_grad_work_doubles(const hs2_plan *plan, int ckpt_every);
 
 /* Forward pass: nsteps steps in place on d_T (T^0 in, T^nsteps out),
