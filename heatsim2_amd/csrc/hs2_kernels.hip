@@ -509,7 +509,11 @@ template <class V, int SEG, int AXIS>
 __global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, const double *Tn,
                                                        const double *Y, double *X, double *surf,
                                                        double dz) {
-  sweep_strided_body<V, SEG, AXIS>(g, view, CW, Tn, Y, X, surf, dz, blockIdx.x, blockIdx.y);
+  // consecutive column blocks share the 128-byte lines of the uint8 class
+  // rows: keep them on one XCD (one L2)
+  const int nb = gridDim.x * gridDim.y;
+  const int r = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
+  sweep_strided_body<V, SEG, AXIS>(g, view, CW, Tn, Y, X, surf, dz, r % gridDim.x, r / gridDim.x);
 }
 
 // ------------------------------------------------------ K0: classify
