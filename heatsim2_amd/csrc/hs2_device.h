@@ -361,6 +361,94 @@ __device__ __forceinline__ void group_solve(SegOf<CPL> &s, double (&x)[CPL], int
   }
 }
 
+// Seg with the right-hand sides kept in the lane's own LDS slots rp[0..N-1]
+// instead of registers (the same elimination; the solution is written back
+// into the slots).  Frees 2N VGPRs in kernels whose occupancy the
+// registers limit.
+template <int N>
+struct SegLds {
+  static_assert(N >= 2, "segment needs two rows");
+  double a[N], c[N];
+  double b0, rprev;
+  double *rp;
+  __device__ __forceinline__ void row(int q, double aq, double bq, double cq, double rq) {
+    if (q == 0) {
+      a[0] = aq;
+      b0 = bq;
+      c[0] = cq;
+      rprev = rq;
+    } else if (q == 1) {
+      const double k = rcp(bq);
+      a[1] = aq * k;
+      c[1] = cq * k;
+      rprev = rq * k;
+    } else {
+      const double k = rcp(bq - aq * c[q - 1]);
+      a[q] = -aq * a[q - 1] * k;
+      c[q] = cq * k;
+      rprev = (rq - aq * rprev) * k;
+    }
+    rp[q] = rprev;
+  }
+  __device__ __forceinline__ void finish(double &As, double &Cs, double &Rs, double &Ae,
+                                         double &Ce, double &Re) {
+    Ae = a[N - 1];
+    Ce = c[N - 1];
+    Re = rprev;
+    if constexpr (N == 2) {
+      const double k = rcp(b0);
+      As = a[0] * k;
+      Cs = c[0] * k;
+      Rs = rp[0] * k;
+    } else {
+      double rnext = rp[N - 2];
+#pragma unroll
+      for (int q = N - 3; q >= 1; q--) {
+        a[q] = a[q] - c[q] * a[q + 1];
+        const double rq = rp[q] - c[q] * rnext;
+        rp[q] = rq;
+        rnext = rq;
+        c[q] = -c[q] * c[q + 1];
+      }
+      const double k = rcp(b0 - c[0] * a[1]);
+      As = a[0] * k;
+      Cs = -c[0] * c[1] * k;
+      Rs = (rp[0] - c[0] * rnext) * k;
+    }
+  }
+};
+
+// group_solve for a SegLds: the solution overwrites the lane's slots.
+template <int CPL, int G>
+__device__ __forceinline__ void group_solve_lds(SegLds<CPL> &s, int lig) {
+  using X = LaneX<G>;
+  double As, Cs, Rs, Ae, Ce, Re;
+  s.finish(As, Cs, Rs, Ae, Ce, Re);
+  if (__any(coupling(As, Cs, Ae, Ce) > kNegligible)) {
+    {
+      const double aL = X::up(Ae, 1, lig), cL = X::up(Ce, 1, lig), rL = X::up(Re, 1, lig);
+      const double aR = X::down(As, 1, lig), cR = X::down(Cs, 1, lig), rR = X::down(Rs, 1, lig);
+      double as = As, cs = Cs, rs = Rs, ae = Ae, ce = Ce, re = Re;
+      pcr_combine(as, cs, rs, aL, cL, rL, Ae, Ce, Re);
+      pcr_combine(ae, ce, re, As, Cs, Rs, aR, cR, rR);
+      As = as; Cs = cs; Rs = rs; Ae = ae; Ce = ce; Re = re;
+    }
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+      const double aL = X::up(As, d, lig), cL = X::up(Cs, d, lig), rL = X::up(Rs, d, lig);
+      const double aR = X::down(As, d, lig), cR = X::down(Cs, d, lig), rR = X::down(Rs, d, lig);
+      const double eaL = X::up(Ae, d, lig), ecL = X::up(Ce, d, lig), erL = X::up(Re, d, lig);
+      const double eaR = X::down(Ae, d, lig), ecR = X::down(Ce, d, lig), erR = X::down(Re, d, lig);
+      pcr_combine(As, Cs, Rs, aL, cL, rL, aR, cR, rR);
+      pcr_combine(Ae, Ce, Re, eaL, ecL, erL, eaR, ecR, erR);
+    }
+  }
+#pragma unroll
+  for (int q = 1; q < CPL - 1; q++) s.rp[q] = s.rp[q] - s.a[q] * Rs - s.c[q] * Re;
+  s.rp[0] = Rs;
+  s.rp[CPL - 1] = Re;
+}
+
 // readout of surface_temperature.py:31-36 for the first two z-layers.
 __device__ __forceinline__ double surface_readout(double T0, double T1, double dz) {
   const double dz2 = dz * dz;

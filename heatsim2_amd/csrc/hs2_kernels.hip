@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
 // SEG consecutive cells per line, recomputes the x-axis coefficients from
 // the class bytes and solves like k_sweep_x.  LPB lines per block.  SRC adds
 // the volumetric source terms (only on the rare steps that carry one).
-template <class V, int SEG, int G, int LPB, bool SRC>
+template <class V, int SEG, int G, int LPB, bool SRC, bool FULL = false>
 __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__restrict__ Tn,
                                               double *__restrict__ Ts,
                                               const uint8_t *__restrict__ vmap,
@@ -321,7 +321,6 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   const int base = lbase[lq][0];
   const int i0 = lig * SEG;
   typename V::Raw rxs[SEG];
-  double rh[SEG];
   if constexpr (SEG % 4 == 0) {
     if ((nx & 3) == 0 && i0 + SEG <= nx) {
 #pragma unroll
@@ -339,35 +338,68 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
 #pragma unroll
     for (int q = 0; q < SEG; q++) rxs[q] = view.fetch(2, base + min(i0 + q, nx - 1));
   }
+  if constexpr (SEG >= 2) {
+    // right-hand sides stay in the lane's LDS slots (contiguous: SEG divides
+    // 16), which also receive the solution; coefficients are resolved as the
+    // rows stream by
+    SegLds<SEG> seg;
+    seg.rp = rbuf + lq * RS + i0 + (i0 >> 4);
+    Entry ec = view.resolve(tabx, rxs[0]);
+    const double g_after = LaneX<G>::down(ec.g, 1, lig);  // next lane's first face
 #pragma unroll
-  for (int q = 0; q < SEG; q++) {
-    const int x = min(i0 + q, nx - 1);
-    rh[q] = rbuf[lq * RS + x + (x >> 4)];
-  }
-  Entry ex[SEG + 1];
+    for (int q = 0; q < SEG; q++) {
+      const int x = i0 + q;
+      Entry en;
+      if (q + 1 < SEG) en = view.resolve(tabx, rxs[q + 1]);
+      else en.g = g_after;
+      if constexpr (FULL) {  // every lane's cells lie inside its (existing) line
+        const double gxm = (q == 0 && lig == 0) ? 0.0 : ec.g;
+        const double gxp = (q == SEG - 1 && lig == G - 1) ? 0.0 : en.g;
+        const double am = -0.5 * ec.w * gxm, cp = -0.5 * ec.w * gxp;
+        seg.row(q, am, ec.M - am - cp, cp, seg.rp[q]);
+      } else {
+        const bool valid = lvalid && x < nx;
+        const double gxm = x > 0 ? ec.g : 0.0;
+        const double gxp = x < nx - 1 ? en.g : 0.0;
+        const double am = -0.5 * ec.w * gxm, cp = -0.5 * ec.w * gxp;
+        const double rq = valid ? seg.rp[q] : 0.0;
+        seg.row(q, valid ? am : 0.0, valid ? ec.M - am - cp : 1.0, valid ? cp : 0.0, rq);
+      }
+      ec = en;
+    }
+    group_solve_lds<SEG, G>(seg, lig);
+  } else {
+    Entry ex[SEG + 1];
 #pragma unroll
-  for (int q = 0; q < SEG; q++) ex[q] = view.resolve(tabx, rxs[q]);
-  ex[SEG].g = LaneX<G>::down(ex[0].g, 1, lig);
-  SegOf<SEG> seg;
+    for (int q = 0; q < SEG; q++) ex[q] = view.resolve(tabx, rxs[q]);
+    ex[SEG].g = LaneX<G>::down(ex[0].g, 1, lig);
+    double rh[SEG];
 #pragma unroll
-  for (int q = 0; q < SEG; q++) {
-    const int x = i0 + q;
-    const bool valid = lvalid && x < nx;
-    const double gxm = x > 0 ? ex[q].g : 0.0;
-    const double gxp = x < nx - 1 ? ex[q + 1].g : 0.0;
-    const double am = -0.5 * ex[q].w * gxm, cp = -0.5 * ex[q].w * gxp;
-    seg.row(q, valid ? am : 0.0, valid ? ex[q].M - am - cp : 1.0, valid ? cp : 0.0,
-            valid ? rh[q] : 0.0);
-  }
-  double xs[SEG];
-  group_solve<SEG, G>(seg, xs, lig);
-  // ---- write back through LDS so that the global stores are coalesced
-  // (each lane owns SEG consecutive cells; its reads of rbuf above touched
-  // only those cells)
+    for (int q = 0; q < SEG; q++) {
+      const int x = min(i0 + q, nx - 1);
+      rh[q] = rbuf[lq * RS + x + (x >> 4)];
+    }
+    SegOf<SEG> seg;
 #pragma unroll
-  for (int q = 0; q < SEG; q++) {
-    const int x = i0 + q;
-    if (x < nx) rbuf[lq * RS + x + (x >> 4)] = xs[q];
+    for (int q = 0; q < SEG; q++) {
+      const int x = i0 + q;
+      const bool valid = lvalid && x < nx;
+      const double gxm = x > 0 ? ex[q].g : 0.0;
+      const double gxp = x < nx - 1 ? ex[q + 1].g : 0.0;
+      const double am = -0.5 * ex[q].w * gxm, cp = -0.5 * ex[q].w * gxp;
+      seg.row(q, valid ? am : 0.0, valid ? ex[q].M - am - cp : 1.0, valid ? cp : 0.0,
+              valid ? rh[q] : 0.0);
+    }
+    double xs[SEG];
+    group_solve<SEG, G>(seg, xs, lig);
+    // write back through LDS so that the global stores are coalesced (each
+    // lane owns SEG consecutive cells; its reads of rbuf above touched only
+    // those cells)
+#pragma unroll
+    for (int q = 0; q < SEG; q++) {
+      const int x = i0 + q;
+      if (x < nx) rbuf[lq * RS + x + (x >> 4)] = xs[q];
+    }
   }
   __syncthreads();
   if ((nx & 3) == 0) {
@@ -389,15 +421,15 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   }
 }
 
-template <class V, int SEG, int G, int LPB, bool SRC>
+template <class V, int SEG, int G, int LPB, bool SRC, bool FULL>
 __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
                                                     const double *__restrict__ Tn,
                                                     double *__restrict__ Ts,
                                                     const uint8_t *__restrict__ vmap,
                                                     const double *__restrict__ vtab_g,
                                                     const double *__restrict__ sfield) {
-  sweep_x2_body<V, SEG, G, LPB, SRC>(g, view, Tn, Ts, vmap, vtab_g, sfield,
-                                     xcd_remap(blockIdx.x, gridDim.x));
+  sweep_x2_body<V, SEG, G, LPB, SRC, FULL>(g, view, Tn, Ts, vmap, vtab_g, sfield,
+                                           xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------- y / z sweeps
@@ -586,12 +618,17 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, const double *Tn, doubl
   const bool src = vmap || sfield;
   const size_t shm =
       sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + (src ? kMaxVtab : 0) + LPB * RS);
+  // FULL: lines fill the lanes exactly and every block holds existing lines
+  const bool full = g.nx == SEG * G && nlines % LPB == 0;
   if (src)
-    hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, true>), dim3(blocks), dim3(G * LPB), shm, s, g,
-                       v, Tn, Ts, vmap, vtab, sfield);
+    hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, true, false>), dim3(blocks), dim3(G * LPB), shm,
+                       s, g, v, Tn, Ts, vmap, vtab, sfield);
+  else if (full)
+    hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, true>), dim3(blocks), dim3(G * LPB), shm,
+                       s, g, v, Tn, Ts, vmap, vtab, sfield);
   else
-    hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false>), dim3(blocks), dim3(G * LPB), shm, s, g,
-                       v, Tn, Ts, vmap, vtab, sfield);
+    hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, false>), dim3(blocks), dim3(G * LPB),
+                       shm, s, g, v, Tn, Ts, vmap, vtab, sfield);
   return hipGetLastError();
 }
 

@@ -203,11 +203,14 @@ def test_deterministic(hs):
     assert torch.equal(outs[0], outs[1])
 
 
-@pytest.mark.parametrize("shape", [(20, 48, 200), (130, 9, 40), (9, 300, 12)],
+@pytest.mark.parametrize("shape", [(20, 48, 200), (130, 9, 40), (9, 300, 12), (4, 8, 256),
+                                   (6, 4, 64), (2, 16, 32)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_every_launch_shape_matches_oracle(hs, shape):
     """Every launch shape the autotuner may pick (x: cells per lane x lanes per
-    line; y, z: segment length x lines per block) gives the oracle's result."""
+    line; y, z: segment length x lines per block) gives the oracle's result.
+    The last three grids fill the x lanes exactly with whole blocks of lines
+    (the unmasked x path) for several shapes."""
     from heatsim2_amd import _lib, autotune
     case = _random_case(shape, seed=7 + sum(shape))
     case["nsteps"] = 2
