@@ -141,7 +141,9 @@ __device__ __forceinline__ void adj_line_body(Grid g, V view, int CW, const doub
 template <class V, int SEG, int AXIS>
 __global__ __launch_bounds__(256) void k_adj_line(Grid g, V view, int CW, const double *R,
                                                   int scale_m, double *U) {
-  adj_line_body<V, SEG, AXIS>(g, view, CW, R, scale_m, U, blockIdx.x, blockIdx.y);
+  const int nb = gridDim.x * gridDim.y;
+  const int r = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
+  adj_line_body<V, SEG, AXIS>(g, view, CW, R, scale_m, U, r % gridDim.x, r / gridDim.x);
 }
 
 // ------------------------------------------------------ adjoint stencil

@@ -48,7 +48,9 @@ struct BatchMap {
 // Instance, local block and step of this block; false if the instance has
 // nothing to do under the launch's rule (the whole block returns).
 __device__ __forceinline__ bool batch_locate(const BatchMap &m, BatchInst &I, int &local, int &t) {
-  const int gb = blockIdx.x;
+  // consecutive blocks of an instance share cache lines (uint8 rows, halos):
+  // keep them on one XCD
+  const int gb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = m.blk_inst[gb];
   local = gb - m.blk0[i];
   I = m.inst[i];

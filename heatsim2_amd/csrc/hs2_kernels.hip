@@ -437,7 +437,7 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
 // line stride nx*ny.  Solves (M - La/2) X = M Y - (La/2) T^n along each line.
 // X may alias Y or T^n: a block owns whole lines, and all of its loads
 // happen before its first barrier and all of its stores after its last.
-template <class V, int SEG, int AXIS>
+template <class V, int SEG, int AXIS, bool FULL = false>
 __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const double *Tn,
                                                    const double *Y, double *X, double *surf,
                                                    double dz, int bx, int by) {
@@ -464,15 +464,28 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
   // ---- loads: T^n over the segment plus one halo cell each side, Y, entries
   double T[SEG + 2], Yv[SEG];
   typename V::Raw raw[SEG + 1];
+  if constexpr (FULL) {  // segments tile the line exactly: only the halo cells clamp
+    const int b0 = base + t0 * stride;
+    T[0] = Tn[base + max(t0 - 1, 0) * stride];
 #pragma unroll
-  for (int q = 0; q < SEG + 2; q++) {
-    const int t = min(max(t0 + q - 1, 0), L - 1);
-    T[q] = Tn[base + t * stride];
+    for (int q = 1; q <= SEG; q++) T[q] = Tn[b0 + (q - 1) * stride];
+    T[SEG + 1] = Tn[base + min(t0 + SEG, L - 1) * stride];
+#pragma unroll
+    for (int q = 0; q < SEG; q++) Yv[q] = Y[b0 + q * stride];
+#pragma unroll
+    for (int q = 0; q < SEG; q++) raw[q] = view.fetch(axis, b0 + q * stride);
+    raw[SEG] = view.fetch(axis, base + min(t0 + SEG, L - 1) * stride);
+  } else {
+#pragma unroll
+    for (int q = 0; q < SEG + 2; q++) {
+      const int t = min(max(t0 + q - 1, 0), L - 1);
+      T[q] = Tn[base + t * stride];
+    }
+#pragma unroll
+    for (int q = 0; q < SEG; q++) Yv[q] = Y[base + min(t0 + q, L - 1) * stride];
+#pragma unroll
+    for (int q = 0; q <= SEG; q++) raw[q] = view.fetch(axis, base + min(t0 + q, L - 1) * stride);
   }
-#pragma unroll
-  for (int q = 0; q < SEG; q++) Yv[q] = Y[base + min(t0 + q, L - 1) * stride];
-#pragma unroll
-  for (int q = 0; q <= SEG; q++) raw[q] = view.fetch(axis, base + min(t0 + q, L - 1) * stride);
 
   // ---- rows, normalised to unit diagonal (independent of each other, so the
   // divisions overlap), then reduced to the segment's two end rows
@@ -482,17 +495,24 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
   for (int q = 0; q < SEG; q++) {
     const int t = t0 + q;
     const Entry ep = view.resolve(lds, raw[q + 1]);
-    const double gm = t >= 1 ? en.g : 0.0;
-    const double gp = t + 1 <= L - 1 ? ep.g : 0.0;
+    // FULL: only the line's first and last cells have an outer face
+    const double gm = (FULL ? (q > 0 || p > 0) : t >= 1) ? en.g : 0.0;
+    const double gp = (FULL ? (q < SEG - 1 || t0 + SEG < L) : t + 1 <= L - 1) ? ep.g : 0.0;
     const double Tc = T[q + 1];
     const double M = en.M, w = en.w;
     const double rhs = M * Yv[q] - 0.5 * w * (gm * (T[q] - Tc) + gp * (T[q + 2] - Tc));
     const double am = -0.5 * w * gm, cp = -0.5 * w * gp;
     const double inv = 1.0 / (M - am - cp);
-    const bool ok = valid && t < L;
-    a[q] = ok ? am * inv : 0.0;
-    c[q] = ok ? cp * inv : 0.0;
-    r[q] = ok ? rhs * inv : 0.0;
+    if constexpr (FULL) {
+      a[q] = am * inv;
+      c[q] = cp * inv;
+      r[q] = rhs * inv;
+    } else {
+      const bool ok = valid && t < L;
+      a[q] = ok ? am * inv : 0.0;
+      c[q] = ok ? cp * inv : 0.0;
+      r[q] = ok ? rhs * inv : 0.0;
+    }
     en = ep;
   }
   double As, Cs, Rs, Ae, Ce, Re;
@@ -537,7 +557,7 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
   if (surf && axis == 0 && p == 0 && L >= 2) surf[base] = surface_readout(x[0], x[1], dz);
 }
 
-template <class V, int SEG, int AXIS>
+template <class V, int SEG, int AXIS, bool FULL>
 __global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, const double *Tn,
                                                        const double *Y, double *X, double *surf,
                                                        double dz) {
@@ -545,7 +565,8 @@ __global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, c
   // rows: keep them on one XCD (one L2)
   const int nb = gridDim.x * gridDim.y;
   const int r = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
-  sweep_strided_body<V, SEG, AXIS>(g, view, CW, Tn, Y, X, surf, dz, r % gridDim.x, r / gridDim.x);
+  sweep_strided_body<V, SEG, AXIS, FULL>(g, view, CW, Tn, Y, X, surf, dz, r % gridDim.x,
+                                         r / gridDim.x);
 }
 
 // ------------------------------------------------------ K0: classify
@@ -695,12 +716,24 @@ static hipError_t launch_s_t(const Grid &g, const V &v, int axis, int CW, const 
   const int other = axis == 1 ? g.nz : g.ny;
   dim3 grid((g.nx + CW - 1) / CW, other);
   const size_t lds = strided_lds_bytes_t<V>(v, axis, P, CW);
-  if (axis == 1)
-    hipLaunchKernelGGL((k_sweep_strided<V, SEG, 1>), grid, dim3(P * CW), lds, s, g, v, CW, Tn, Y,
-                       X, surf, dz);
-  else
-    hipLaunchKernelGGL((k_sweep_strided<V, SEG, 0>), grid, dim3(P * CW), lds, s, g, v, CW, Tn, Y,
-                       X, surf, dz);
+  // FULL: the segments tile the line and the column blocks tile nx
+  const bool full = P * SEG == L && g.nx % CW == 0;
+  const dim3 block(P * CW);
+  if (axis == 1) {
+    if (full)
+      hipLaunchKernelGGL((k_sweep_strided<V, SEG, 1, true>), grid, block, lds, s, g, v, CW, Tn, Y,
+                         X, surf, dz);
+    else
+      hipLaunchKernelGGL((k_sweep_strided<V, SEG, 1, false>), grid, block, lds, s, g, v, CW, Tn,
+                         Y, X, surf, dz);
+  } else {
+    if (full)
+      hipLaunchKernelGGL((k_sweep_strided<V, SEG, 0, true>), grid, block, lds, s, g, v, CW, Tn, Y,
+                         X, surf, dz);
+    else
+      hipLaunchKernelGGL((k_sweep_strided<V, SEG, 0, false>), grid, block, lds, s, g, v, CW, Tn,
+                         Y, X, surf, dz);
+  }
   return hipGetLastError();
 }
 
