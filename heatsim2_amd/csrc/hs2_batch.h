@@ -48,9 +48,9 @@ struct BatchMap {
 // Instance, local block and step of this block; false if the instance has
 // nothing to do under the launch's rule (the whole block returns).
 __device__ __forceinline__ bool batch_locate(const BatchMap &m, BatchInst &I, int &local, int &t) {
-  // consecutive blocks of an instance share cache lines (uint8 rows, halos):
-  // keep them on one XCD
-  const int gb = xcd_remap(blockIdx.x, gridDim.x);
+  // (no XCD regrouping here: instances finish at different steps, and whole
+  // instances per XCD would leave XCDs idle)
+  const int gb = blockIdx.x;
   const int i = m.blk_inst[gb];
   local = gb - m.blk0[i];
   I = m.inst[i];
