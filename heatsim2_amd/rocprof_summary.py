@@ -9,7 +9,7 @@
         dispatch), each scaled by what the same counter reads for a plain
         device copy of B bytes
 
-Kernels are attributed to sweeps by name (the strided kernel's last
+Kernels are attributed to sweeps by name (the strided kernel's third
 template argument is its axis).  The last line of the traffic
 summary is ``json: {...}`` with the per-sweep traffic in bytes; bench.py
 reads it.
@@ -44,12 +44,12 @@ def short_name(name):
     return name.strip()
 
 
-_STRIDED = re.compile(r"k_sweep_strided<[^<>]*,\s*(\d+)>")
+_STRIDED = re.compile(r"k_sweep_strided<[^,<>]+,\s*\d+,\s*(\d+)")
 
 
 def sweep_of(name):
     """'x', 'y' or 'z' for a sweep kernel's name, else None.  The strided
-    kernel carries its axis (1: y, 0: z) as its last template argument."""
+    kernel carries its axis (1: y, 0: z) as its third template argument."""
     if "k_sweep_x" in name:
         return "x"
     m = _STRIDED.search(name)

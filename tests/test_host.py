@@ -164,8 +164,8 @@ def test_launch_shape_candidates_fit_the_kernels():
 
 
 def test_rocprof_summary_attribution_and_committed_traffic():
-    assert rocprof_summary.sweep_of("void hs2::k_sweep_strided<hs2::ClassView, 16, 1>(x)") == "y"
-    assert rocprof_summary.sweep_of("void hs2::k_sweep_strided<hs2::ClassView, 16, 0>(x)") == "z"
+    assert rocprof_summary.sweep_of("void hs2::k_sweep_strided<hs2::ClassView, 16, 1, true>(x)") == "y"
+    assert rocprof_summary.sweep_of("void hs2::k_sweep_strided<hs2::ClassView, 8, 0, false>(x)") == "z"
     assert rocprof_summary.sweep_of("void hs2::k_sweep_x2<hs2::ClassView, 16, 16, 4, false>") == "x"
     assert rocprof_summary.sweep_of("hs2::k_classify") is None
     tr = rocprof_summary.read_traffic(os.path.join(ROOT, "profiles",
