@@ -162,12 +162,17 @@ __device__ __forceinline__ double LaneX<G>::down(double v, int s, int lig) {
   return lig + s < G ? x : 0.0;
 }
 
-// 1/x from the hardware reciprocal refined by one Newton step (error within a
-// couple of ulp of the correctly rounded quotient; x is a well-scaled
-// positive pivot here, never 0, inf or denormal).
+// 1/x from the hardware reciprocal refined by two Newton steps.  Measured on
+// gfx950 over 2^20 values in [2^-30, 2^30]: v_rcp_f64 alone is good to
+// ~2^-24.4 relative, one step leaves up to ~20 x 2^-53, two steps 1 x 2^-53
+// (as IEEE division).  The one-step error accumulated visibly over a
+// thousand time steps.  x is a well-scaled positive pivot here, never 0,
+// inf or denormal.
 __device__ __forceinline__ double rcp(double x) {
   double r = __builtin_amdgcn_rcp(x);
-  const double e = __builtin_fma(-x, r, 1.0);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
   return __builtin_fma(r, e, r);
 }
 
