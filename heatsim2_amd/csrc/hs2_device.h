@@ -103,6 +103,39 @@ struct DenseView {
   }
 };
 
+// ------------------------------------------------------ buffer access
+// Buffer descriptors (gfx9 family: third dword 0x00020000) give loads and
+// stores a 32-bit byte offset from a scalar base; accesses beyond
+// num_records read 0 and are dropped.
+using BufRsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ BufRsrc make_buf(const void *p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ double2 buf_load_d2(BufRsrc r, unsigned off) {
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ double buf_load_d(BufRsrc r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ uint32_t buf_load_u32(BufRsrc r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ __forceinline__ uint32_t buf_load_u8(BufRsrc r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_store_d2(double2 v, BufRsrc r, unsigned off) {
+  __builtin_amdgcn_raw_buffer_store_b128(
+      __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v), r, off, 0, 0);
+}
+// the four bytes of a class word, lowest first
+__device__ __forceinline__ void unpack4(uint32_t w, uint32_t (&r)[4]) {
+  r[0] = w & 0xffu;
+  r[1] = (w >> 8) & 0xffu;
+  r[2] = (w >> 16) & 0xffu;
+  r[3] = w >> 24;
+}
+
 // Logical block index that keeps consecutive logical blocks on one XCD.
 // Hardware hands block b to XCD b % 8; give each XCD a contiguous range.
 __device__ __forceinline__ int xcd_remap(int b, int nblocks) {

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
 // SEG consecutive cells per line, recomputes the x-axis coefficients from
 // the class bytes and solves like k_sweep_x.  LPB lines per block.  SRC adds
 // the volumetric source terms (only on the rare steps that carry one).
-template <class V, int SEG, int G, int LPB, bool SRC, bool FULL = false>
+template <class V, int SEG, int G, int LPB, bool SRC, bool FULL = false, bool BUF = false>
 __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__restrict__ Tn,
                                               double *__restrict__ Ts,
                                               const uint8_t *__restrict__ vmap,
@@ -153,6 +153,16 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   const int nlines = nz * ny;
   const int line0 = blk * LPB;
   const int tid = threadIdx.x;
+  // BUF (class maps, field < 4 GiB): loads and stores through buffer
+  // descriptors sized to the arrays
+  [[maybe_unused]] BufRsrc bT, bS, bC[3];
+  if constexpr (BUF) {
+    const unsigned ncell = (unsigned)nlines * (unsigned)nx;
+    bT = make_buf(Tn, ncell * 8u);
+    bS = make_buf(Ts, ncell * 8u);
+#pragma unroll
+    for (int a = 0; a < 3; a++) bC[a] = make_buf(view.cls[a], ncell);
+  }
   if (tid < LPB) {
     const int line = line0 + tid;
     const int lc = line < nlines ? line : nlines - 1;
@@ -196,6 +206,28 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
         const int lb = li[b] < LPB ? li[b] : LPB - 1;
         const int x0 = xq[b] << 2;
         const int b0 = lbase[lb][0];
+        const int xl = x0 > 0 ? x0 - 1 : 0, xr = x0 + 4 < nx ? x0 + 4 : nx - 1;
+        if constexpr (BUF) {
+          // 32-bit byte offsets from buffer descriptors: no 64-bit address math
+          const unsigned o[5] = {(unsigned)(b0 + x0), (unsigned)(lbase[lb][1] + x0),
+                                 (unsigned)(lbase[lb][2] + x0), (unsigned)(lbase[lb][3] + x0),
+                                 (unsigned)(lbase[lb][4] + x0)};
+          double (*dst[5])[4] = {&T[b], &Tym[b], &Typ[b], &Tzm[b], &Tzp[b]};
+#pragma unroll
+          for (int k = 0; k < 5; k++) {
+            const double2 v0 = buf_load_d2(bT, o[k] * 8u);
+            const double2 v1 = buf_load_d2(bT, o[k] * 8u + 16u);
+            (*dst[k])[0] = v0.x; (*dst[k])[1] = v0.y; (*dst[k])[2] = v1.x; (*dst[k])[3] = v1.y;
+          }
+          TL[b] = buf_load_d(bT, (unsigned)(b0 + xl) * 8u);
+          TR[b] = buf_load_d(bT, (unsigned)(b0 + xr) * 8u);
+          unpack4(buf_load_u32(bC[2], o[0]), rx[b]);
+          rxn[b] = buf_load_u8(bC[2], (unsigned)(b0 + xr));
+          unpack4(buf_load_u32(bC[1], o[0]), ry[b]);
+          unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
+          unpack4(buf_load_u32(bC[0], o[0]), rz[b]);
+          unpack4(buf_load_u32(bC[0], o[4]), rzp[b]);
+        } else {
         const double2 *p0 = reinterpret_cast<const double2 *>(Tn + b0 + x0);
         const double2 *p1 = reinterpret_cast<const double2 *>(Tn + lbase[lb][1] + x0);
         const double2 *p2 = reinterpret_cast<const double2 *>(Tn + lbase[lb][2] + x0);
@@ -207,14 +239,15 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
         v = p2[0]; Typ[b][0] = v.x; Typ[b][1] = v.y; v = p2[1]; Typ[b][2] = v.x; Typ[b][3] = v.y;
         v = p3[0]; Tzm[b][0] = v.x; Tzm[b][1] = v.y; v = p3[1]; Tzm[b][2] = v.x; Tzm[b][3] = v.y;
         v = p4[0]; Tzp[b][0] = v.x; Tzp[b][1] = v.y; v = p4[1]; Tzp[b][2] = v.x; Tzp[b][3] = v.y;
-        TL[b] = Tn[b0 + (x0 > 0 ? x0 - 1 : 0)];
-        TR[b] = Tn[b0 + (x0 + 4 < nx ? x0 + 4 : nx - 1)];
+        TL[b] = Tn[b0 + xl];
+        TR[b] = Tn[b0 + xr];
         view.fetch4(2, b0 + x0, rx[b]);
-        rxn[b] = view.fetch(2, b0 + (x0 + 4 < nx ? x0 + 4 : nx - 1));
+        rxn[b] = view.fetch(2, b0 + xr);
         view.fetch4(1, b0 + x0, ry[b]);
         view.fetch4(1, lbase[lb][2] + x0, ryp[b]);
         view.fetch4(0, b0 + x0, rz[b]);
         view.fetch4(0, lbase[lb][4] + x0, rzp[b]);
+        }
       }
 #pragma unroll
       for (int b = 0; b < UQ; b++) {
@@ -321,7 +354,16 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   const int base = lbase[lq][0];
   const int i0 = lig * SEG;
   typename V::Raw rxs[SEG];
-  if constexpr (SEG % 4 == 0) {
+  if constexpr (BUF) {
+    static_assert(SEG % 4 == 0, "BUF path loads class words");
+#pragma unroll
+    for (int q = 0; q < SEG; q += 4) {
+      typename V::Raw t4[4];
+      unpack4(buf_load_u32(bC[2], (unsigned)(base + i0 + q)), t4);
+#pragma unroll
+      for (int u = 0; u < 4; u++) rxs[q + u] = t4[u];
+    }
+  } else if constexpr (SEG % 4 == 0) {
     if ((nx & 3) == 0 && i0 + SEG <= nx) {
 #pragma unroll
       for (int q = 0; q < SEG; q += 4) {
@@ -408,9 +450,15 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
       const int l2 = e / qpl, x0 = (e - l2 * qpl) << 2;
       if (line0 + l2 < nlines) {
         const double *src = rbuf + l2 * RS + x0 + (x0 >> 4);
-        double2 *dst = reinterpret_cast<double2 *>(Ts + lbase[l2][0] + x0);
-        dst[0] = make_double2(src[0], src[1]);
-        dst[1] = make_double2(src[2], src[3]);
+        if constexpr (BUF) {
+          const unsigned o = (unsigned)(lbase[l2][0] + x0) * 8u;
+          buf_store_d2(make_double2(src[0], src[1]), bS, o);
+          buf_store_d2(make_double2(src[2], src[3]), bS, o + 16u);
+        } else {
+          double2 *dst = reinterpret_cast<double2 *>(Ts + lbase[l2][0] + x0);
+          dst[0] = make_double2(src[0], src[1]);
+          dst[1] = make_double2(src[2], src[3]);
+        }
       }
     }
   } else {
@@ -428,8 +476,9 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
                                                     const uint8_t *__restrict__ vmap,
                                                     const double *__restrict__ vtab_g,
                                                     const double *__restrict__ sfield) {
-  sweep_x2_body<V, SEG, G, LPB, SRC, FULL>(g, view, Tn, Ts, vmap, vtab_g, sfield,
-                                           xcd_remap(blockIdx.x, gridDim.x));
+  constexpr bool BUF = FULL && SEG % 4 == 0 && std::is_same<V, ClassView>::value;
+  sweep_x2_body<V, SEG, G, LPB, SRC, FULL, BUF>(g, view, Tn, Ts, vmap, vtab_g, sfield,
+                                                xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------- y / z sweeps
@@ -640,7 +689,8 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, const double *Tn, doubl
   const size_t shm =
       sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + (src ? kMaxVtab : 0) + LPB * RS);
   // FULL: lines fill the lanes exactly and every block holds existing lines
-  const bool full = g.nx == SEG * G && nlines % LPB == 0;
+  const bool full = g.nx == SEG * G && nlines % LPB == 0 &&
+                    (size_t)nlines * g.nx * sizeof(double) < (size_t(1) << 32);
   if (src)
     hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, true, false>), dim3(blocks), dim3(G * LPB), shm,
                        s, g, v, Tn, Ts, vmap, vtab, sfield);
