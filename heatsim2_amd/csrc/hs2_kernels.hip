@@ -51,7 +51,6 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
   const bool lvalid = line < nlines;
   const int lc = lvalid ? line : nlines - 1;
   const int k = lc / ny, j = lc - k * ny;
-  const int sxy = nx * ny;
   const int base = lc * nx;
   const int bym = (j > 0 ? lc - 1 : lc) * nx, byp = (j < ny - 1 ? lc + 1 : lc) * nx;
   const int bzm = (k > 0 ? lc - ny : lc) * nx, bzp = (k < nz - 1 ? lc + ny : lc) * nx;
@@ -139,7 +138,6 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
                                               const double *__restrict__ sfield, int blk) {
   constexpr int NT = G * LPB;
   constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;  // padded LDS row (doubles)
-  constexpr int U = 4;                                // cells per thread per batch
   extern __shared__ double lds[];
   __shared__ int lbase[LPB][5];    // own, y-1, y+1, z-1, z+1 row bases
   __shared__ double lmask[LPB][4];  // 1/0: neighbour exists at y-1, y+1, z-1, z+1
