@@ -99,31 +99,10 @@ __device__ __forceinline__ void adj_line_body(Grid g, V view, int CW, const doub
   }
   double As, Cs, Rs, Ae, Ce, Re;
   segment_reduce<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
-  const int NR = 2 * P;
   ra[(2 * p) * CW + cl] = As; rc[(2 * p) * CW + cl] = Cs; rr[(2 * p) * CW + cl] = Rs;
   ra[(2 * p + 1) * CW + cl] = Ae; rc[(2 * p + 1) * CW + cl] = Ce; rr[(2 * p + 1) * CW + cl] = Re;
   __syncthreads();
-  for (int d = 1; d < NR; d <<= 1) {
-    double na[2], nc[2], nr[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int row = 2 * p + h;
-      double aa = ra[row * CW + cl], cc = rc[row * CW + cl], r0 = rr[row * CW + cl];
-      const int rl = row - d >= 0 ? row - d : row, rh = row + d < NR ? row + d : row;
-      const double m0 = row - d >= 0 ? 1.0 : 0.0, m1 = row + d < NR ? 1.0 : 0.0;
-      const double aL = m0 * ra[rl * CW + cl], cL = m0 * rc[rl * CW + cl], rL = m0 * rr[rl * CW + cl];
-      const double aR = m1 * ra[rh * CW + cl], cR = m1 * rc[rh * CW + cl], rR = m1 * rr[rh * CW + cl];
-      pcr_combine(aa, cc, r0, aL, cL, rL, aR, cR, rR);
-      na[h] = aa; nc[h] = cc; nr[h] = r0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int row = 2 * p + h;
-      ra[row * CW + cl] = na[h]; rc[row * CW + cl] = nc[h]; rr[row * CW + cl] = nr[h];
-    }
-    __syncthreads();
-  }
+  lds_pcr_ends(ra, rc, rr, p, P, CW, cl);
   const double xs = rr[(2 * p) * CW + cl], xe = rr[(2 * p + 1) * CW + cl];
   if (!valid) return;
   double x[SEG];

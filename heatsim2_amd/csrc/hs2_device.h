@@ -24,6 +24,7 @@
 namespace hs2 {
 
 constexpr int kWave = 64;
+constexpr int kMaxWaves = 1024 / kWave;  // waves in the largest block
 constexpr int kMaxVtab = 256;
 constexpr int kMaxClass = 256;
 constexpr int kTabStride = 4;  // doubles per class-table entry: M, w, g, pad
@@ -334,6 +335,51 @@ __device__ __forceinline__ double coupling(double As, double Cs, double Ae, doub
   return fmax(fmax(fabs(As), fabs(Cs)), fmax(fabs(Ae), fabs(Ce)));
 }
 
+// PCR across the threads of a block over the 2P end rows of P segments per
+// line, CW lines side by side in LDS (row r of line cl at r*CW + cl); thread
+// (p, cl) owns rows 2p and 2p+1, written, with a barrier passed, on entry.
+// On return rr holds the end rows' solutions.  After the first level every
+// coupling is a near one times one across a whole segment; once all of them
+// are negligible (usual on a diagonally dominant line) the other levels are
+// skipped.  The block-wide decision rides on the level's own barrier: each
+// wave posts its ballot before it, every thread reads all posts after it.
+__device__ __forceinline__ void lds_pcr_ends(double *ra, double *rc, double *rr, int p, int P,
+                                             int CW, int cl) {
+  __shared__ int wave_more[kMaxWaves];
+  const int R = 2 * P;
+  for (int d = 1; d < R; d <<= 1) {
+    double na[2], nc[2], nr[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int row = 2 * p + h;
+      double aa = ra[row * CW + cl], cc = rc[row * CW + cl], r0 = rr[row * CW + cl];
+      const int rl = row - d >= 0 ? row - d : row, rh = row + d < R ? row + d : row;
+      const double m0 = row - d >= 0 ? 1.0 : 0.0, m1 = row + d < R ? 1.0 : 0.0;
+      const double aL = m0 * ra[rl * CW + cl], cL = m0 * rc[rl * CW + cl], rL = m0 * rr[rl * CW + cl];
+      const double aR = m1 * ra[rh * CW + cl], cR = m1 * rc[rh * CW + cl], rR = m1 * rr[rh * CW + cl];
+      pcr_combine(aa, cc, r0, aL, cL, rL, aR, cR, rR);
+      na[h] = aa; nc[h] = cc; nr[h] = r0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int row = 2 * p + h;
+      ra[row * CW + cl] = na[h]; rc[row * CW + cl] = nc[h]; rr[row * CW + cl] = nr[h];
+    }
+    if (d == 1) {
+      const bool more = coupling(na[0], nc[0], na[1], nc[1]) > kNegligible;
+      const unsigned long long bal = __ballot(more);
+      if ((threadIdx.x & (kWave - 1)) == 0) wave_more[threadIdx.x / kWave] = bal != 0ull;
+    }
+    __syncthreads();
+    if (d == 1) {
+      int any = 0;
+      for (int w = 0; w < (int)(blockDim.x + kWave - 1) / kWave; w++) any |= wave_more[w];
+      if (!any) break;
+    }
+  }
+}
+
 // Single-row "segment" (CPL = 1): the row itself, normalised.
 struct Row1 {
   double a, c, r;
@@ -382,14 +428,19 @@ __device__ __forceinline__ void group_solve(SegOf<CPL> &s, double (&x)[CPL], int
         pcr_combine(ae, ce, re, As, Cs, Rs, aR, cR, rR);
         As = as; Cs = cs; Rs = rs; Ae = ae; Ce = ce; Re = re;
       }
+      // after the pairing each coupling is a near one times one across a
+      // whole segment: on a diagonally dominant line these are usually
+      // negligible already, and the remaining levels are skipped
+      if (__any(coupling(As, Cs, Ae, Ce) > kNegligible)) {
 #pragma unroll
-      for (int d = 1; d < G; d <<= 1) {
-        const double aL = X::up(As, d, lig), cL = X::up(Cs, d, lig), rL = X::up(Rs, d, lig);
-        const double aR = X::down(As, d, lig), cR = X::down(Cs, d, lig), rR = X::down(Rs, d, lig);
-        const double eaL = X::up(Ae, d, lig), ecL = X::up(Ce, d, lig), erL = X::up(Re, d, lig);
-        const double eaR = X::down(Ae, d, lig), ecR = X::down(Ce, d, lig), erR = X::down(Re, d, lig);
-        pcr_combine(As, Cs, Rs, aL, cL, rL, aR, cR, rR);
-        pcr_combine(Ae, Ce, Re, eaL, ecL, erL, eaR, ecR, erR);
+        for (int d = 1; d < G; d <<= 1) {
+          const double aL = X::up(As, d, lig), cL = X::up(Cs, d, lig), rL = X::up(Rs, d, lig);
+          const double aR = X::down(As, d, lig), cR = X::down(Cs, d, lig), rR = X::down(Rs, d, lig);
+          const double eaL = X::up(Ae, d, lig), ecL = X::up(Ce, d, lig), erL = X::up(Re, d, lig);
+          const double eaR = X::down(Ae, d, lig), ecR = X::down(Ce, d, lig), erR = X::down(Re, d, lig);
+          pcr_combine(As, Cs, Rs, aL, cL, rL, aR, cR, rR);
+          pcr_combine(Ae, Ce, Re, eaL, ecL, erL, eaR, ecR, erR);
+        }
       }
     }
     x[0] = Rs;
@@ -471,14 +522,16 @@ __device__ __forceinline__ void group_solve_lds(SegLds<CPL> &s, int lig) {
       pcr_combine(ae, ce, re, As, Cs, Rs, aR, cR, rR);
       As = as; Cs = cs; Rs = rs; Ae = ae; Ce = ce; Re = re;
     }
+    if (__any(coupling(As, Cs, Ae, Ce) > kNegligible)) {  // as in group_solve
 #pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-      const double aL = X::up(As, d, lig), cL = X::up(Cs, d, lig), rL = X::up(Rs, d, lig);
-      const double aR = X::down(As, d, lig), cR = X::down(Cs, d, lig), rR = X::down(Rs, d, lig);
-      const double eaL = X::up(Ae, d, lig), ecL = X::up(Ce, d, lig), erL = X::up(Re, d, lig);
-      const double eaR = X::down(Ae, d, lig), ecR = X::down(Ce, d, lig), erR = X::down(Re, d, lig);
-      pcr_combine(As, Cs, Rs, aL, cL, rL, aR, cR, rR);
-      pcr_combine(Ae, Ce, Re, eaL, ecL, erL, eaR, ecR, erR);
+      for (int d = 1; d < G; d <<= 1) {
+        const double aL = X::up(As, d, lig), cL = X::up(Cs, d, lig), rL = X::up(Rs, d, lig);
+        const double aR = X::down(As, d, lig), cR = X::down(Cs, d, lig), rR = X::down(Rs, d, lig);
+        const double eaL = X::up(Ae, d, lig), ecL = X::up(Ce, d, lig), erL = X::up(Re, d, lig);
+        const double eaR = X::down(Ae, d, lig), ecR = X::down(Ce, d, lig), erR = X::down(Re, d, lig);
+        pcr_combine(As, Cs, Rs, aL, cL, rL, aR, cR, rR);
+        pcr_combine(Ae, Ce, Re, eaL, ecL, erL, eaR, ecR, erR);
+      }
     }
   }
 #pragma unroll

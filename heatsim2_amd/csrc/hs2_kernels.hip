@@ -139,8 +139,11 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   constexpr int NT = G * LPB;
   constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;  // padded LDS row (doubles)
   extern __shared__ double lds[];
-  __shared__ int lbase[LPB][5];    // own, y-1, y+1, z-1, z+1 row bases
-  __shared__ double lmask[LPB][4];  // 1/0: neighbour exists at y-1, y+1, z-1, z+1
+  // own, y-1, y+1, z-1, z+1 row bases.  A neighbour row outside the grid is
+  // replaced by the line itself, and a missing x neighbour by the cell
+  // itself: that face's difference T_nb - T_c is then exactly 0, so its
+  // (finite) conductance needs no mask.
+  __shared__ int lbase[LPB][5];
   view.load_tables(lds, -1);
   const int tabpad = (view.lds_doubles(-1) + 1) & ~1;
   double *vtab = lds + tabpad;
@@ -170,10 +173,6 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
     lbase[tid][2] = (j < ny - 1 ? lc + 1 : lc) * nx;
     lbase[tid][3] = (k > 0 ? lc - ny : lc) * nx;
     lbase[tid][4] = (k < nz - 1 ? lc + ny : lc) * nx;
-    lmask[tid][0] = j > 0 ? 1.0 : 0.0;
-    lmask[tid][1] = j < ny - 1 ? 1.0 : 0.0;
-    lmask[tid][2] = k > 0 ? 1.0 : 0.0;
-    lmask[tid][3] = k < nz - 1 ? 1.0 : 0.0;
   }
   __syncthreads();
   const double *tabz = view.table(lds, 0), *taby = view.table(lds, 1), *tabx = view.table(lds, 2);
@@ -181,27 +180,37 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   // ---- phase 1: right-hand sides, lanes along x
   if ((nx & 3) == 0) {
     // quads: 4 consecutive cells per thread, vector loads; quad e = tid + NT*u
-    // of the block's LPB x nx/4 quads, walked incrementally as (l, q)
-    constexpr int UQ = 2;
-    const int qpl = nx >> 2;
+    // of the block's LPB x nx/4 quads, walked incrementally as (l, q).  FQ
+    // (full lines, nx = SEG*G): the quads per line and the trip count are
+    // compile-time and every quad exists, so no bounds checks.
+    constexpr bool FQ = FULL && SEG % 4 == 0;
+    constexpr int UQ = FQ && SEG < 8 ? 1 : 2;
+    const int qpl = FQ ? SEG * G / 4 : nx >> 2;
     const int total = LPB * qpl;
     int l = tid / qpl, qx = tid - (tid / qpl) * qpl;
     const int dl = NT / qpl, dq = NT - (NT / qpl) * qpl;
+#pragma unroll 1
     for (int e0 = 0; e0 < total; e0 += UQ * NT) {
       int li[UQ], xq[UQ];
 #pragma unroll
       for (int b = 0; b < UQ; b++) {
-        li[b] = l;
-        xq[b] = qx;
-        qx += dq;
-        l += dl;
-        if (qx >= qpl) { qx -= qpl; l++; }
+        if constexpr (FQ) {
+          const int e = e0 + b * NT + tid;
+          li[b] = e / qpl;
+          xq[b] = e % qpl;
+        } else {
+          li[b] = l;
+          xq[b] = qx;
+          qx += dq;
+          l += dl;
+          if (qx >= qpl) { qx -= qpl; l++; }
+        }
       }
       double T[UQ][4], Tym[UQ][4], Typ[UQ][4], Tzm[UQ][4], Tzp[UQ][4], TL[UQ], TR[UQ];
       typename V::Raw rx[UQ][4], rxn[UQ], ry[UQ][4], ryp[UQ][4], rz[UQ][4], rzp[UQ][4];
 #pragma unroll
       for (int b = 0; b < UQ; b++) {
-        const int lb = li[b] < LPB ? li[b] : LPB - 1;
+        const int lb = FQ || li[b] < LPB ? li[b] : LPB - 1;
         const int x0 = xq[b] << 2;
         const int b0 = lbase[lb][0];
         const int xl = x0 > 0 ? x0 - 1 : 0, xr = x0 + 4 < nx ? x0 + 4 : nx - 1;
@@ -249,7 +258,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
       }
 #pragma unroll
       for (int b = 0; b < UQ; b++) {
-        const int lb = li[b] < LPB ? li[b] : LPB - 1;
+        const int lb = FQ || li[b] < LPB ? li[b] : LPB - 1;
         const int x0 = xq[b] << 2;
         Entry ex[5];
 #pragma unroll
@@ -262,12 +271,11 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           const double T0 = T[b][q];
           const double tl = q > 0 ? T[b][q - 1] : TL[b];
           const double tr = q < 3 ? T[b][q + 1] : TR[b];
-          const double gxm = (x > 0 ? 1.0 : 0.0) * ex[q].g;
-          const double gxp = (x < nx - 1 ? 1.0 : 0.0) * ex[q + 1].g;
-          const double gym = lmask[lb][0] * view.resolve(taby, ry[b][q]).g;
-          const double gyp = lmask[lb][1] * view.resolve(taby, ryp[b][q]).g;
-          const double gzm = lmask[lb][2] * view.resolve(tabz, rz[b][q]).g;
-          const double gzp = lmask[lb][3] * view.resolve(tabz, rzp[b][q]).g;
+          const double gxm = ex[q].g, gxp = ex[q + 1].g;
+          const double gym = view.resolve(taby, ry[b][q]).g;
+          const double gyp = view.resolve(taby, ryp[b][q]).g;
+          const double gzm = view.resolve(tabz, rz[b][q]).g;
+          const double gzp = view.resolve(tabz, rzp[b][q]).g;
           double expl = 0.5 * (gxm * (tl - T0) + gxp * (tr - T0)) + gym * (Tym[b][q] - T0) +
                         gyp * (Typ[b][q] - T0) + gzm * (Tzm[b][q] - T0) + gzp * (Tzp[b][q] - T0);
           if (SRC) {
@@ -276,7 +284,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           }
           out[q] = ex[q].M * T0 + ex[q].w * expl;
         }
-        if (li[b] < LPB) {
+        if (FQ || li[b] < LPB) {
           double *dst = rbuf + li[b] * RS + x0 + (x0 >> 4);
 #pragma unroll
           for (int q = 0; q < 4; q++) dst[q] = out[q];
@@ -325,14 +333,13 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
       for (int b = 0; b < U; b++) {
         const int lb = li[b] < LPB ? li[b] : LPB - 1;
         const int x = ii[b];
-        const double mxm = x > 0 ? 1.0 : 0.0, mxp = x < nx - 1 ? 1.0 : 0.0;
         const Entry e = view.resolve(tabx, rx[b]);
-        const double gxm = mxm * e.g;
-        const double gxp = mxp * view.resolve(tabx, rxp[b]).g;
-        const double gym = lmask[lb][0] * view.resolve(taby, ry[b]).g;
-        const double gyp = lmask[lb][1] * view.resolve(taby, ryp[b]).g;
-        const double gzm = lmask[lb][2] * view.resolve(tabz, rz[b]).g;
-        const double gzp = lmask[lb][3] * view.resolve(tabz, rzp[b]).g;
+        const double gxm = e.g;
+        const double gxp = view.resolve(tabx, rxp[b]).g;
+        const double gym = view.resolve(taby, ry[b]).g;
+        const double gyp = view.resolve(taby, ryp[b]).g;
+        const double gzm = view.resolve(tabz, rz[b]).g;
+        const double gzp = view.resolve(tabz, rzp[b]).g;
         const double T0 = Tc[b];
         double expl = 0.5 * (gxm * (Txm[b] - T0) + gxp * (Txp[b] - T0)) + gym * (Tym[b] - T0) +
                       gyp * (Typ[b] - T0) + gzm * (Tzm[b] - T0) + gzp * (Tzp[b] - T0);
@@ -564,31 +571,10 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
   }
   double As, Cs, Rs, Ae, Ce, Re;
   segment_reduce<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
-  const int R = 2 * P;
   ra[(2 * p) * CW + cl] = As; rc[(2 * p) * CW + cl] = Cs; rr[(2 * p) * CW + cl] = Rs;
   ra[(2 * p + 1) * CW + cl] = Ae; rc[(2 * p + 1) * CW + cl] = Ce; rr[(2 * p + 1) * CW + cl] = Re;
   __syncthreads();
-  for (int d = 1; d < R; d <<= 1) {
-    double na[2], nc[2], nr[2];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int row = 2 * p + h;
-      double aa = ra[row * CW + cl], cc = rc[row * CW + cl], r0 = rr[row * CW + cl];
-      const int rl = row - d >= 0 ? row - d : row, rh = row + d < R ? row + d : row;
-      const double m0 = row - d >= 0 ? 1.0 : 0.0, m1 = row + d < R ? 1.0 : 0.0;
-      const double aL = m0 * ra[rl * CW + cl], cL = m0 * rc[rl * CW + cl], rL = m0 * rr[rl * CW + cl];
-      const double aR = m1 * ra[rh * CW + cl], cR = m1 * rc[rh * CW + cl], rR = m1 * rr[rh * CW + cl];
-      pcr_combine(aa, cc, r0, aL, cL, rL, aR, cR, rR);
-      na[h] = aa; nc[h] = cc; nr[h] = r0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int row = 2 * p + h;
-      ra[row * CW + cl] = na[h]; rc[row * CW + cl] = nc[h]; rr[row * CW + cl] = nr[h];
-    }
-    __syncthreads();
-  }
+  lds_pcr_ends(ra, rc, rr, p, P, CW, cl);
   const double xs = rr[(2 * p) * CW + cl], xe = rr[(2 * p + 1) * CW + cl];
   if (!valid) return;
   double x[SEG];

@@ -204,16 +204,19 @@ def test_deterministic(hs):
 
 
 @pytest.mark.parametrize("shape", [(20, 48, 200), (130, 9, 40), (9, 300, 12), (4, 8, 256),
-                                   (6, 4, 64), (2, 16, 32)],
+                                   (6, 4, 64), (2, 16, 32), (13, 24, 128), (7, 16, 64),
+                                   (5, 8, 512)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_every_launch_shape_matches_oracle(hs, shape):
     """Every launch shape the autotuner may pick (x: cells per lane x lanes per
-    line; y, z: segment length x lines per block) gives the oracle's result.
-    The last three grids fill the x lanes exactly with whole blocks of lines
-    (the unmasked x path) for several shapes."""
+    line; y, z: segment length x lines per block) gives the oracle's result,
+    over steps with and without source terms.  From (4, 8, 256) on, the grids
+    fill the x lanes exactly with whole blocks of lines (the unmasked x path)
+    for several shapes."""
     from heatsim2_amd import _lib, autotune
     case = _random_case(shape, seed=7 + sum(shape))
-    case["nsteps"] = 2
+    # the stepped source lasts through t = 3 ms: steps 4 and 5 have none
+    case["nsteps"] = 6
     args = to_package_args(hs, case)
     P, S = hs.setup(*args["setup"])
     ref = oracle.run_ref(case)
