@@ -363,8 +363,9 @@ def main():
                                   "layer h=250 W/m^2K, flash %.1f J/m^2, dt=0.5 ms, FP64, surface "
                                   "readout every step" % (nz, ny, nx, case["flash_J_m2"]),
                       "cells": n, "parallelism": "single GPU",
-                      "launch": {k: tuned[k] for k in ("x_cpl", "x_lanes", "x_lines", "y_seg",
-                                                       "y_lines", "z_seg", "z_lines", "mode")}},
+                      "launch": {k: tuned[k] for k in ("x_cpl", "x_lanes", "x_lines",
+                                                       "x_lines_y", "y_seg", "y_lines", "z_seg",
+                                                       "z_lines", "mode")}},
            "roofline": roofline}
     del T, hist
     P.release()

@@ -60,7 +60,7 @@ def load():
     L.hs2_plan_set_dense.restype = i32
     L.hs2_plan_expand.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.hs2_plan_expand.restype = i32
-    L.hs2_plan_set_config.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32]
+    L.hs2_plan_set_config.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32, i32]
     L.hs2_plan_set_config.restype = i32
     L.hs2_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.hs2_plan_info.restype = i32
@@ -141,8 +141,8 @@ def surface_temperature_device(T, dz):
 
 
 def plan_info(plan):
-    info = (ctypes.c_int * 11)()
+    info = (ctypes.c_int * 12)()
     check("hs2_plan_info", load().hs2_plan_info(plan, info))
     keys = ("mode", "classes_z", "classes_y", "classes_x", "x_cpl", "x_lanes", "y_seg",
-            "y_lines", "z_seg", "z_lines", "x_lines")
+            "y_lines", "z_seg", "z_lines", "x_lines", "x_lines_y")
     return dict(zip(keys, list(info)))

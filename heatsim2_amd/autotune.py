@@ -1,9 +1,10 @@
 """Launch-shape autotuner for the three sweep kernels of a plan.
 
-For the x sweep it chooses (cells per lane, lanes per line); for the y and z
-sweeps (cells per thread segment, lines per block) -- i.e. how much of each
-line is eliminated Thomas-style in registers and how much goes to the
-parallel cyclic reduction across lanes / threads.  Candidates are timed on
+For the x sweep it chooses (cells per lane, lanes per line, lines per block,
+rows of the block's line tile); for the y and z sweeps (cells per thread
+segment, lines per block) -- i.e. how much of each line is eliminated
+Thomas-style in registers and how much goes to the parallel cyclic
+reduction across lanes / threads.  Candidates are timed on
 the device with HIP events (hs2_run_timed) on a scratch copy of the field;
 the fastest of each sweep is set on the plan.  Results are cached per
 (device name, grid shape, coefficient mode) in memory and, if writable, in
@@ -33,9 +34,19 @@ MAX_BLOCK = 512
 _memo = {}
 
 
+def x_tiles(lines):
+    """Row counts of a two-phase block's line tile: 0 (lines in order) and
+    the proper divisors of `lines` up to 8."""
+    if lines == 0:
+        return (0,)
+    return (0,) + tuple(t for t in (1, 2, 4, 8) if t < lines and lines % t == 0)
+
+
 def x_candidates(nx):
+    """x-sweep shapes worth timing: (cells per lane, lanes, lines, tile rows)."""
     out = [c for c in X_COMBOS if c[0] * c[1] >= nx and c[0] * c[1] < 2 * nx + 16]
-    return out or [c for c in X_COMBOS if c[0] * c[1] >= nx][:1]
+    out = out or [c for c in X_COMBOS if c[0] * c[1] >= nx][:1]
+    return [c + (t,) for c in out for t in x_tiles(c[2])]
 
 
 def strided_candidates(L):
@@ -84,7 +95,8 @@ def _time(L, plan, T, steps):
 
 def tune(ADI_params, T=None, steps=4, use_cache=True, verbose=False):
     """Pick and set the fastest launch shapes for this plan; returns
-    {"x": (cpl, lanes, lines), "y": (seg, lines), "z": (seg, lines), "ms": [x, y, z]}."""
+    {"x": (cpl, lanes, lines, tile rows), "y": (seg, lines), "z": (seg, lines),
+    "ms": [x, y, z]}."""
     L = _lib.load()
     plan = ADI_params.plan
     nz, ny, nx = ADI_params.shape
@@ -100,7 +112,7 @@ def tune(ADI_params, T=None, steps=4, use_cache=True, verbose=False):
     if T is None:
         T = torch.full(ADI_params.shape, 293.0, dtype=torch.float64, device=dev)
     scratch = T.detach().clone()
-    cur = {"x": (info["x_cpl"], info["x_lanes"], info["x_lines"]),
+    cur = {"x": (info["x_cpl"], info["x_lanes"], info["x_lines"], info["x_lines_y"]),
            "y": (info["y_seg"], info["y_lines"]),
            "z": (info["z_seg"], info["z_lines"])}
     best_ms = [None, None, None]
@@ -131,7 +143,8 @@ def tune(ADI_params, T=None, steps=4, use_cache=True, verbose=False):
 
 
 def _apply(L, plan, cfg, strict=True):
-    rc = L.hs2_plan_set_config(plan, int(cfg["x"][0]), int(cfg["x"][1]), int(cfg["x"][2]),
+    x = list(cfg["x"]) + [0] * (4 - len(cfg["x"]))
+    rc = L.hs2_plan_set_config(plan, int(x[0]), int(x[1]), int(x[2]), int(x[3]),
                                int(cfg["y"][0]), int(cfg["y"][1]), int(cfg["z"][0]),
                                int(cfg["z"][1]))
     if strict:

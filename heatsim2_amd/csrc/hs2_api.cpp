@@ -112,6 +112,7 @@ bool strided_ok(int L, hs2::SConfig c) {
 }
 
 bool x_ok(int nx, hs2::XConfig c) {
+  if (c.ty != 0 && (c.lpb == 0 || c.ty < 0 || c.ty >= c.lpb || c.lpb % c.ty != 0)) return false;
   static const int combos[][3] = {
       {1, 16, 0}, {2, 16, 0}, {4, 16, 0}, {8, 16, 0}, {16, 16, 0}, {2, 32, 0}, {4, 32, 0},
       {8, 32, 0}, {16, 32, 0}, {2, 64, 0}, {4, 64, 0}, {8, 64, 0}, {16, 64, 0},
@@ -430,7 +431,7 @@ int hs2_plan_create(const hs2_grid_desc *grid, hs2_plan **out) {
   if ((long long)(G.nz + 1) * G.ny * G.nx >= (1LL << 31) ||
       (long long)G.nz * G.ny * (G.nx + 1) >= (1LL << 31))
     return fail(HS2_E_UNSUPPORTED, "hs2_plan_create: grid of %lld cells exceeds 2^31 indices", n);
-  const hs2::XConfig xc = hs2::x_config_for(G.nx);
+  const hs2::XConfig xc = hs2::x_config_for(G.nx, G.nz);
   if (!xc.cpl) return fail(HS2_E_UNSUPPORTED, "hs2_plan_create: nx=%d > 1024 unsupported", G.nx);
   hs2_plan *p = new hs2_plan();
   p->grid = G;
@@ -623,17 +624,18 @@ int hs2_plan_info(const hs2_plan *p, int *info) {
   info[7] = p->scfg[1].cw;
   info[8] = p->scfg[0].seg;
   info[9] = p->scfg[0].cw;
+  info[11] = p->xcfg.ty;
   return HS2_OK;
 }
 
-int hs2_plan_set_config(hs2_plan *p, int x_cpl, int x_lanes, int x_lines, int y_seg, int y_lines,
-                        int z_seg, int z_lines) {
+int hs2_plan_set_config(hs2_plan *p, int x_cpl, int x_lanes, int x_lines, int x_lines_y,
+                        int y_seg, int y_lines, int z_seg, int z_lines) {
   if (!p) return fail(HS2_E_ARG, "hs2_plan_set_config: null plan");
-  const hs2::XConfig xc{x_cpl, x_lanes, x_lines};
+  const hs2::XConfig xc{x_cpl, x_lanes, x_lines, x_lines_y};
   const hs2::SConfig yc{y_seg, y_lines}, zc{z_seg, z_lines};
   if (!x_ok(p->grid.nx, xc))
-    return fail(HS2_E_ARG, "hs2_plan_set_config: x config %dx%dx%d invalid for nx=%d", x_cpl,
-                x_lanes, x_lines, p->grid.nx);
+    return fail(HS2_E_ARG, "hs2_plan_set_config: x config %dx%dx%d (%d rows) invalid for nx=%d",
+                x_cpl, x_lanes, x_lines, x_lines_y, p->grid.nx);
   if (!strided_ok(p->grid.ny, yc))
     return fail(HS2_E_ARG, "hs2_plan_set_config: y config seg=%d lines=%d invalid for ny=%d",
                 y_seg, y_lines, p->grid.ny);

@@ -135,15 +135,18 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
                                               double *__restrict__ Ts,
                                               const uint8_t *__restrict__ vmap,
                                               const double *__restrict__ vtab_g,
-                                              const double *__restrict__ sfield, int blk) {
+                                              const double *__restrict__ sfield, int blk,
+                                              int ty) {
   constexpr int NT = G * LPB;
   constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;  // padded LDS row (doubles)
   extern __shared__ double lds[];
   // own, y-1, y+1, z-1, z+1 row bases.  A neighbour row outside the grid is
   // replaced by the line itself, and a missing x neighbour by the cell
   // itself: that face's difference T_nb - T_c is then exactly 0, so its
-  // (finite) conductance needs no mask.
+  // (finite) conductance needs no mask.  lok: the line exists (lines past
+  // the grid's end load a clamped line and store nothing).
   __shared__ int lbase[LPB][5];
+  __shared__ bool lok[LPB];
   view.load_tables(lds, -1);
   const int tabpad = (view.lds_doubles(-1) + 1) & ~1;
   double *vtab = lds + tabpad;
@@ -165,9 +168,28 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
     for (int a = 0; a < 3; a++) bC[a] = make_buf(view.cls[a], ncell);
   }
   if (tid < LPB) {
-    const int line = line0 + tid;
-    const int lc = line < nlines ? line : nlines - 1;
-    const int k = lc / ny, j = lc - k * ny;
+    // ty = 0: the block's lines follow each other in (k, j) order.  ty > 0:
+    // they tile LPB/ty planes x ty rows, so that lines share z neighbour rows
+    // as well as y ones (fewer rows fetched per line).
+    int k, j;
+    bool ok;
+    if (ty > 0) {
+      const int tz = LPB / ty, nbj = (ny + ty - 1) / ty;
+      const int kt = blk / nbj, jt = blk - kt * nbj;
+      k = kt * tz + tid / ty;
+      j = jt * ty + tid % ty;
+      ok = k < nz && j < ny;
+      k = min(k, nz - 1);
+      j = min(j, ny - 1);
+    } else {
+      const int line = line0 + tid;
+      ok = line < nlines;
+      const int lc = ok ? line : nlines - 1;
+      k = lc / ny;
+      j = lc - k * ny;
+    }
+    const int lc = k * ny + j;
+    lok[tid] = ok;
     lbase[tid][0] = lc * nx;
     lbase[tid][1] = (j > 0 ? lc - 1 : lc) * nx;
     lbase[tid][2] = (j < ny - 1 ? lc + 1 : lc) * nx;
@@ -355,7 +377,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
 
   // ---- phase 2: G lanes x SEG cells per line
   const int lq = tid / G, lig = tid % G;
-  const bool lvalid = line0 + lq < nlines;
+  const bool lvalid = lok[lq];
   const int base = lbase[lq][0];
   const int i0 = lig * SEG;
   typename V::Raw rxs[SEG];
@@ -453,7 +475,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
     const int qpl = nx >> 2;
     for (int e = tid; e < LPB * qpl; e += NT) {
       const int l2 = e / qpl, x0 = (e - l2 * qpl) << 2;
-      if (line0 + l2 < nlines) {
+      if (FULL || lok[l2]) {
         const double *src = rbuf + l2 * RS + x0 + (x0 >> 4);
         if constexpr (BUF) {
           const unsigned o = (unsigned)(lbase[l2][0] + x0) * 8u;
@@ -469,7 +491,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   } else {
     for (int e = tid; e < LPB * nx; e += NT) {
       const int l2 = e / nx, x = e - l2 * nx;
-      if (line0 + l2 < nlines) Ts[lbase[l2][0] + x] = rbuf[l2 * RS + x + (x >> 4)];
+      if (lok[l2]) Ts[lbase[l2][0] + x] = rbuf[l2 * RS + x + (x >> 4)];
     }
   }
 }
@@ -480,10 +502,10 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
                                                     double *__restrict__ Ts,
                                                     const uint8_t *__restrict__ vmap,
                                                     const double *__restrict__ vtab_g,
-                                                    const double *__restrict__ sfield) {
+                                                    const double *__restrict__ sfield, int ty) {
   constexpr bool BUF = FULL && SEG % 4 == 0 && std::is_same<V, ClassView>::value;
   sweep_x2_body<V, SEG, G, LPB, SRC, FULL, BUF>(g, view, Tn, Ts, vmap, vtab_g, sfield,
-                                                xcd_remap(blockIdx.x, gridDim.x));
+                                                xcd_remap(blockIdx.x, gridDim.x), ty);
 }
 
 // ---------------------------------------------------------- y / z sweeps
@@ -663,40 +685,47 @@ static hipError_t launch_x_t(const Grid &g, const V &v, const double *Tn, double
 }
 
 template <class V, int SEG, int G, int LPB>
-static hipError_t launch_x2_t(const Grid &g, const V &v, const double *Tn, double *Ts,
+static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *Tn, double *Ts,
                               const uint8_t *vmap, const double *vtab, const double *sfield,
                               hipStream_t s) {
   const int nlines = g.nz * g.ny;
-  const int blocks = (nlines + LPB - 1) / LPB;
+  const int tz = ty > 0 ? LPB / ty : 0;
+  const int blocks = ty > 0 ? ((g.nz + tz - 1) / tz) * ((g.ny + ty - 1) / ty)
+                            : (nlines + LPB - 1) / LPB;
   constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;
   const bool src = vmap || sfield;
   const size_t shm =
       sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + (src ? kMaxVtab : 0) + LPB * RS);
   // FULL: lines fill the lanes exactly and every block holds existing lines
-  const bool full = g.nx == SEG * G && nlines % LPB == 0 &&
+  const bool whole = ty > 0 ? g.nz % tz == 0 && g.ny % ty == 0 : nlines % LPB == 0;
+  const bool full = g.nx == SEG * G && whole &&
                     (size_t)nlines * g.nx * sizeof(double) < (size_t(1) << 32);
   if (src)
     hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, true, false>), dim3(blocks), dim3(G * LPB), shm,
-                       s, g, v, Tn, Ts, vmap, vtab, sfield);
+                       s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
   else if (full)
     hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, true>), dim3(blocks), dim3(G * LPB), shm,
-                       s, g, v, Tn, Ts, vmap, vtab, sfield);
+                       s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
   else
     hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, false>), dim3(blocks), dim3(G * LPB),
-                       shm, s, g, v, Tn, Ts, vmap, vtab, sfield);
+                       shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
   return hipGetLastError();
 }
 
-XConfig x_config_for(int nx) {
-  // two-phase kernel, 16 lanes per line where a line fits 16 x 16 cells
-  if (nx <= 16) return {1, 16, 8};
-  if (nx <= 32) return {2, 16, 8};
-  if (nx <= 64) return {4, 16, 8};
-  if (nx <= 128) return {8, 16, 8};
-  if (nx <= 256) return {16, 16, 8};
-  if (nx <= 512) return {16, 32, 4};
-  if (nx <= 1024) return {16, 64, 2};
-  return {0, 0, 0};
+XConfig x_config_for(int nx, int nz) {
+  // two-phase kernel, 16 lanes per line where a line fits 16 x 16 cells;
+  // blocks of 8 or more lines as tiles 2 rows wide (when there are planes
+  // enough to stack them)
+  XConfig c{0, 0, 0, 0};
+  if (nx <= 16) c = {1, 16, 8};
+  else if (nx <= 32) c = {2, 16, 8};
+  else if (nx <= 64) c = {4, 16, 8};
+  else if (nx <= 128) c = {8, 16, 8};
+  else if (nx <= 256) c = {16, 16, 16};
+  else if (nx <= 512) c = {16, 32, 4};
+  else if (nx <= 1024) c = {16, 64, 2};
+  if (c.lpb >= 8 && 2 * nz >= c.lpb) c.ty = 2;
+  return c;
 }
 
 template <class V>
@@ -712,7 +741,7 @@ static hipError_t launch_x_v(XConfig cfg, const Grid &g, const V &v, const doubl
 #undef HS2_X
 #define HS2_X2(C, GG, LL)                                                          \
   if (cfg.lpb == LL && cfg.cpl == C && cfg.g == GG)                                \
-    return launch_x2_t<V, C, GG, LL>(g, v, Tn, Ts, vmap, vtab, sfield, s);
+    return launch_x2_t<V, C, GG, LL>(g, v, cfg.ty, Tn, Ts, vmap, vtab, sfield, s);
   HS2_X2(1, 16, 8) HS2_X2(2, 16, 8) HS2_X2(4, 16, 8) HS2_X2(8, 16, 8) HS2_X2(16, 16, 8)
   HS2_X2(4, 16, 16) HS2_X2(8, 16, 16) HS2_X2(16, 16, 4) HS2_X2(16, 16, 16)
   HS2_X2(8, 32, 4) HS2_X2(16, 32, 4) HS2_X2(16, 32, 2) HS2_X2(16, 64, 2) HS2_X2(8, 64, 2)
@@ -846,7 +875,7 @@ __global__ __launch_bounds__(G *LPB) void kb_sweep_x2(BatchMap m, DenseView v, c
   const double *sf = (src && t == I.src_step) ? src + I.off : nullptr;
   sweep_x2_body<DenseView, SEG, G, LPB, true>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off),
                                               Tn + I.off, Ts + I.off, nullptr, nullptr, sf,
-                                              local);
+                                              local, 0);
 }
 
 template <int SEG, int AXIS>

@@ -7,13 +7,14 @@
 namespace hs2 {
 
 struct XConfig {  // x sweep: CPL cells per lane, G lanes per line, LPB lines per
-  int cpl, g, lpb;  // block of the two-phase kernel (0: single-phase kernel)
+  int cpl, g, lpb;  // block of the two-phase kernel (0: single-phase kernel);
+  int ty = 0;       // two-phase: the block's lines as LPB/TY planes x TY rows (0: in order)
 };
 struct SConfig {  // strided sweep: SEG cells per thread, CW lines per block
   int seg, cw;
 };
 
-XConfig x_config_for(int nx);  // {0,0}: unsupported (nx > 1024)
+XConfig x_config_for(int nx, int nz);  // {0,0}: unsupported (nx > 1024)
 
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const ClassView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,

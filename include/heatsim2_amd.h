@@ -94,14 +94,17 @@ int hs2_plan_expand(hs2_plan *plan, double *d_M, double *d_w, double *d_gz, doub
  * x_cpl consecutive cells per x-line (x_cpl*x_lanes >= nx; x_lanes in
  * {16,32,64}, x_cpl in {1,2,4,8,16}); x_lines > 0 selects the two-phase
  * kernel with x_lines lines per block (right-hand sides staged in LDS),
- * x_lines = 0 the single-phase one.  y and z sweeps: segments of *_seg cells
- * per thread, *_lines lines per block (segments*lines <= 512).  No upstream
+ * x_lines = 0 the single-phase one.  x_lines_y (two-phase only): 0 puts a
+ * block's lines one after the other in (z, y) order; a proper divisor of
+ * x_lines tiles them as x_lines/x_lines_y planes x x_lines_y rows, which
+ * share more neighbour rows.  y and z sweeps: segments of *_seg cells per
+ * thread, *_lines lines per block (segments*lines <= 512).  No upstream
  * counterpart. */
-int hs2_plan_set_config(hs2_plan *plan, int x_cpl, int x_lanes, int x_lines, int y_seg,
-                        int y_lines, int z_seg, int z_lines);
+int hs2_plan_set_config(hs2_plan *plan, int x_cpl, int x_lanes, int x_lines, int x_lines_y,
+                        int y_seg, int y_lines, int z_seg, int z_lines);
 
-/* info[0..10] = mode (0 none, 1 class maps, 2 dense), classes per axis z/y/x,
- * x_cpl, x_lanes, y_seg, y_lines, z_seg, z_lines, x_lines. */
+/* info[0..11] = mode (0 none, 1 class maps, 2 dense), classes per axis z/y/x,
+ * x_cpl, x_lanes, y_seg, y_lines, z_seg, z_lines, x_lines, x_lines_y. */
 int hs2_plan_info(const hs2_plan *plan, int *info);
 
 /* One full Douglas step T^n -> T^{n+1} (x, y, z substeps).

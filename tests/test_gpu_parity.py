@@ -209,10 +209,11 @@ def test_deterministic(hs):
                          ids=lambda s: "x".join(map(str, s)))
 def test_every_launch_shape_matches_oracle(hs, shape):
     """Every launch shape the autotuner may pick (x: cells per lane x lanes per
-    line; y, z: segment length x lines per block) gives the oracle's result,
-    over steps with and without source terms.  From (4, 8, 256) on, the grids
-    fill the x lanes exactly with whole blocks of lines (the unmasked x path)
-    for several shapes."""
+    line x lines per block x rows of the block's line tile; y, z: segment
+    length x lines per block) gives the oracle's result, over steps with and
+    without source terms.  From (4, 8, 256) on, the grids fill the x lanes
+    exactly (the unmasked x path where the tiles also cover the planes and
+    rows; ragged tiles elsewhere, e.g. 13 planes)."""
     from heatsim2_amd import _lib, autotune
     case = _random_case(shape, seed=7 + sum(shape))
     # the stepped source lasts through t = 3 ms: steps 4 and 5 have none
@@ -222,7 +223,7 @@ def test_every_launch_shape_matches_oracle(hs, shape):
     ref = oracle.run_ref(case)
     rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
     info = _lib.plan_info(P.plan)
-    base = {"x": (info["x_cpl"], info["x_lanes"], info["x_lines"]),
+    base = {"x": (info["x_cpl"], info["x_lanes"], info["x_lines"], info["x_lines_y"]),
             "y": (info["y_seg"], info["y_lines"]),
             "z": (info["z_seg"], info["z_lines"])}
     nz, ny, nx = shape

@@ -157,7 +157,10 @@ def test_launch_shape_candidates_fit_the_kernels():
     for nx in (1, 3, 16, 17, 100, 256, 300, 512, 1000, 1024):
         cands = autotune.x_candidates(nx)
         assert cands and all(c[0] * c[1] >= nx for c in cands)
-        assert all(c in autotune.X_COMBOS for c in cands)
+        assert all(c[:3] in autotune.X_COMBOS for c in cands)
+        # tile rows: 0, or a proper divisor of the two-phase kernel's lines
+        assert all(c[3] == 0 or (c[2] > 0 and c[3] < c[2] and c[2] % c[3] == 0) for c in cands)
+        assert {c[:3] for c in cands if c[3]} == {c[:3] for c in cands if c[2] > 1}
     for L in (1, 2, 5, 64, 100, 256, 517, 1024):
         for seg, cw in autotune.strided_candidates(L):
             assert -(-L // seg) * cw <= autotune.MAX_BLOCK
