@@ -100,7 +100,103 @@ def assemble(shape, d, dt, materials, boundaries, maps):
     return coef
 
 
+def _curved_geometry(shape, d, curv_y, curv_x):
+    """Cell edge lengths and volumes of curvature mode, crank_nicolson.pyx:
+    409-424: with top-surface curvature kappa, an edge at depth k*dz is
+    (1 + k*kappa*dz) times its nominal length; a cell uses the mean of its top
+    and bottom edges and the volume |dx_mean * dy_mean * dz|."""
+    nz = shape[0]
+    dz, dy, dx = d
+    k = np.arange(nz, dtype=np.float64)[:, None, None]
+    cy = np.asarray(curv_y, dtype=np.float64)[None]
+    cx = np.asarray(curv_x, dtype=np.float64)[None]
+    dy_top = (1.0 + k * cy * dz) * dy
+    dy_bot = (1.0 + (k + 1.0) * cy * dz) * dy
+    dx_top = (1.0 + k * cx * dz) * dx
+    dx_bot = (1.0 + (k + 1.0) * cx * dz) * dx
+    dy_mean = (dy_top + dy_bot) / 2.0
+    dx_mean = (dx_top + dx_bot) / 2.0
+    vol = np.abs(dx_mean * dy_mean * dz)
+    return dict(dy_top=dy_top, dy_bot=dy_bot, dx_top=dx_top, dx_bot=dx_bot, dy_mean=dy_mean,
+                dx_mean=dx_mean, vol=vol)
+
+
+def assemble_curved(shape, d, dt, materials, boundaries, maps, curv_y, curv_x):
+    """Per-cell coefficients of curvature mode (n x 8: M, D, gzm, gzp, gym,
+    gyp, gxm, gxp) and the cell volumes, multiplied through by each cell's
+    volume as the upstream writes them (crank_nicolson.pyx:388-457):
+
+      heat flow = qz_m*dx_top*dy_top - qz_p*dx_bot*dy_bot
+                  + (qy_m - qy_p)*dx_mean*dz + (qx_m - qx_p)*dy_mean*dz + S*V,
+      time term = rho*c*V/dt,
+
+    the face laws evaluated with the cell's own dx_mean, dy_mean in place of
+    dx, dy (:433-434) and the neighbour-k fallback of the uncurved path.
+    A cell's faces use its own geometry, so the coupling of two cells across
+    a y or x face differs from each side.  The uncurved assembly (in C)
+    validates the maps and face laws first."""
+    nz, ny, nx = shape
+    dz, dy, dx = d
+    assemble(shape, d, dt, materials, boundaries, maps)  # validation only
+    m, bz, by, bx = [np.asarray(a, dtype=np.int64) for a in maps[:4]]
+    geo = _curved_geometry(shape, d, curv_y, curv_x)
+    fixed = np.array([bool(mt[0]) for mt in materials])
+    kvec = np.array([[mt[1], mt[2], mt[3]] for mt in materials], dtype=np.float64)
+    rhoc = np.array([mt[4] * mt[5] for mt in materials], dtype=np.float64)
+    kind = np.array([b[0] for b in boundaries])
+    hval = np.array([b[1] for b in boundaries], dtype=np.float64)
+    comp = ~fixed[m]
+
+    def alpha(b, mm, mp, axis, dlen):
+        """flux law q = alpha*(T_p - T_m) of face class b between the
+        (fallback) materials mm, mp, with spacing dlen"""
+        a = np.zeros(b.shape)
+        cond = (kind[b] == BND_CONDUCTING) | (kind[b] == BND_ANISO)
+        thin = kind[b] == BND_THIN
+        kmean = (kvec[mm, axis] + kvec[mp, axis]) * 0.5
+        a[cond] = -(kmean * (1.0 / dlen))[cond]
+        a[thin] = -hval[b][thin]
+        return a
+
+    def faces(axis, bmin, bplus, area_m, area_p, dlen):
+        # neighbour materials with the fallback to the cell's own
+        lo = np.roll(m, 1, axis=axis)
+        hi = np.roll(m, -1, axis=axis)
+        idx = np.arange(shape[axis]).reshape([-1 if a == axis else 1 for a in range(3)])
+        lo_ok = (idx > 0) & ~fixed[lo]
+        hi_ok = (idx < shape[axis] - 1) & ~fixed[hi]
+        mm = np.where(lo_ok, lo, m)
+        mp = np.where(hi_ok, hi, m)
+        gm = -alpha(bmin, mm, m, axis, dlen) * area_m
+        gp = -alpha(bplus, m, mp, axis, dlen) * area_p
+        return np.where(comp, gm, 0.0), np.where(comp, gp, 0.0)
+
+    dzv = np.full(shape, dz)
+    gzm, gzp = faces(0, bz[:-1], bz[1:], geo["dx_top"] * geo["dy_top"],
+                     geo["dx_bot"] * geo["dy_bot"], dzv)
+    gym, gyp = faces(1, by[:, :-1], by[:, 1:], geo["dx_mean"] * dz, geo["dx_mean"] * dz,
+                     geo["dy_mean"])
+    gxm, gxp = faces(2, bx[:, :, :-1], bx[:, :, 1:], geo["dy_mean"] * dz, geo["dy_mean"] * dz,
+                     geo["dx_mean"])
+    vol = np.broadcast_to(geo["vol"], shape)
+    M = np.where(comp, rhoc[m] * vol * (1.0 / dt), 1.0)
+    D = np.where(comp, vol, 0.0)
+    coef = np.stack([M, D, gzm, gzp, gym, gyp, gxm, gxp], axis=-1).reshape(-1, 8)
+    return np.ascontiguousarray(coef), np.ascontiguousarray(vol)
+
+
+def case_volume(case):
+    """The cells' volumes: dz*dy*dx, or per cell in curvature mode."""
+    if case.get("curvature") is not None:
+        return _curved_geometry(case["shape"], case["d"], *case["curvature"])["vol"] * \
+            np.ones(case["shape"])
+    return case["d"][0] * case["d"][1] * case["d"][2]
+
+
 def assemble_case(case):
+    if case.get("curvature") is not None:
+        return assemble_curved(case["shape"], case["d"], case["dt"], case["materials"],
+                               case["boundaries"], case["maps"], *case["curvature"])[0]
     return assemble(case["shape"], case["d"], case["dt"], case["materials"], case["boundaries"],
                     case["maps"])
 
@@ -177,8 +273,9 @@ def thomas(a, b, c, r):
 
 def source_field(volumetric, vmap, t, dt, volume, zgrid=None):
     """Volumetric power density S (W/m^3) at call time t.
-    Restates alternatingdirection_c_pyx.pyx:294-386 (volume is the scalar
-    dz*dy*dx of the uncurved grid, crank_nicolson.pyx:261)."""
+    Restates alternatingdirection_c_pyx.pyx:294-386 (volume: the scalar
+    dz*dy*dx of the uncurved grid, crank_nicolson.pyx:261, or the per-cell
+    volumes of curvature mode, :310-316)."""
     S = np.zeros(vmap.shape)
     for idx, entry in enumerate(volumetric):
         kind = entry[0]
@@ -192,7 +289,10 @@ def source_field(volumetric, vmap, t, dt, volume, zgrid=None):
                 S[sel] = w
         elif kind == IMPULSE_POINT_SOURCE_JOULES:
             if t == entry[1]:
-                S[sel] = entry[2] / (volume * dt)
+                if np.ndim(volume):
+                    S[sel] = entry[2] / (np.asarray(volume)[sel] * dt)
+                else:
+                    S[sel] = entry[2] / (volume * dt)
         elif kind == SPATIALLY_Z_DECAYING_TEMPORAL_IMPULSE:
             (_, t_imp, direc, offset, zg, ddz, jpm2, ell) = entry
             if t == t_imp:
@@ -230,7 +330,7 @@ def run_ref(case, nsteps=None, structured=True):
     step, with sources evaluated at t = t0 + n*dt.  Returns list of fields."""
     coef = assemble_case(case)
     shape = case["shape"]
-    vol = case["d"][0] * case["d"][1] * case["d"][2]
+    vol = case_volume(case)
     zg = zgrid(shape, case["d"][0])
     plan = RefPlan(shape, coef) if structured else None
     T = np.array(case["T0"], dtype=np.float64)

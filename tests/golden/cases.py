@@ -68,6 +68,22 @@ def case_tiny_mixed():
                 maps=(m, bz, by, bx, v), T0=T0, matrices=True)
 
 
+def case_tiny_curved(seed=13):
+    """Curvature mode (crank_nicolson.pyx:388-457): the tiny_mixed layout
+    (two conducting materials + FIXED, thin z, y and x faces) with seeded
+    top-surface curvatures of both signs in y and x, up to +-60 /m so that the
+    edges change by up to a fifth over the depth, and impulse, stepped and
+    point sources (the point source uses the per-cell volumes)."""
+    c = case_tiny_mixed()
+    ny, nx = c["shape"][1], c["shape"][2]
+    rng = np.random.default_rng(seed)
+    cy = rng.uniform(-60.0, 60.0, (ny, nx))
+    cx = rng.uniform(-60.0, 60.0, (ny, nx))
+    cy[0, 0] = cx[0, 0] = 0.0
+    c.update(name="tiny_curved", curvature=(cy, cx))
+    return c
+
+
 def case_tiny_aniso():
     """Diagonal-tensor k with the anisotropic face law, thin layers, FIXED
     cells, a scalar-k material on isotropic faces elsewhere is not mixed in

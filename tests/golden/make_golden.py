@@ -66,9 +66,12 @@ def ref_setup(hs, case, materials=None):
     nz, ny, nx = case["shape"]
     dz, dy, dx = case["d"]
     m, bz, by, bx, v = case["maps"]
+    curv = case.get("curvature")
+    kw = {} if curv is None else dict(top_surface_y_curvatures=np.array(curv[0], "d"),
+                                      top_surface_x_curvatures=np.array(curv[1], "d"))
     with _refload.quiet():
         P, S = hs.setup(0.0, 0.0, 0.0, dz, dy, dx, nz, ny, nx, case["dt"], mats, bnds, vol,
-                        m, bz, by, bx, v)
+                        m, bz, by, bx, v, **kw)
     return P, S, vol
 
 
@@ -88,8 +91,11 @@ def case_json(case):
 
 def maps_dict(case):
     m, bz, by, bx, v = case["maps"]
-    return dict(material_elements=m, boundary_z_elements=bz, boundary_y_elements=by,
-                boundary_x_elements=bx, volumetric_elements=v)
+    out = dict(material_elements=m, boundary_z_elements=bz, boundary_y_elements=by,
+               boundary_x_elements=bx, volumetric_elements=v)
+    if case.get("curvature") is not None:
+        out["curv_y"], out["curv_x"] = case["curvature"]
+    return out
 
 
 def gen_matrix_case(hs, case):
@@ -253,14 +259,16 @@ def gen_grad_case(hs, nvox=10, rel=3e-3):
 
 def main(argv):
     hs = _refload.load_reference()
-    todo = argv or ["tiny_mixed", "tiny_aniso", "steel_cube40", "two_layer_mini",
-                    "laminate_mini", "grad_tiny"]
+    todo = argv or ["tiny_mixed", "tiny_aniso", "tiny_curved", "steel_cube40",
+                    "two_layer_mini", "laminate_mini", "grad_tiny"]
     for name in todo:
         t = time.time()
         if name == "tiny_mixed":
             out = gen_matrix_case(hs, cases.case_tiny_mixed())
         elif name == "tiny_aniso":
             out = gen_matrix_case(hs, cases.case_tiny_aniso())
+        elif name == "tiny_curved":
+            out = gen_matrix_case(hs, cases.case_tiny_curved())
         elif name == "steel_cube40":
             out = gen_trajectory_case(hs, cases.case_steel_cube(40, 200), zprofile_at=(20, 20))
         elif name == "two_layer_mini":

@@ -32,6 +32,8 @@ def load_golden(name):
     case["maps"] = (z["material_elements"], z["boundary_z_elements"], z["boundary_y_elements"],
                     z["boundary_x_elements"], z["volumetric_elements"])
     case["T0"] = z["T0"]
+    if "curv_y" in z.files:
+        case["curvature"] = (z["curv_y"], z["curv_x"])
     case["fixture"] = z
     return case
 
@@ -73,4 +75,6 @@ def to_package_args(hs, case):
     m, bz, by, bx, v = case["maps"]
     setup_args = (0.0, 0.0, 0.0, dz, dy, dx, nz, ny, nx, case["dt"], tuple(materials),
                   tuple(boundaries), tuple(volumetric), m, bz, by, bx, v)
+    if case.get("curvature") is not None:
+        setup_args += (np.array(case["curvature"][0], "d"), np.array(case["curvature"][1], "d"))
     return dict(setup=setup_args, vmap=v, volumetric=tuple(volumetric))

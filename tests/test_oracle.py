@@ -10,7 +10,7 @@ import scipy.sparse as sp
 import oracle
 from hs2_cases import cases, load_golden
 
-MATRIX_CASES = ("tiny_mixed", "tiny_aniso")
+MATRIX_CASES = ("tiny_mixed", "tiny_aniso", "tiny_curved")
 TRAJ_CASES = ("steel_cube40", "two_layer_mini", "laminate_mini")
 
 
@@ -66,6 +66,26 @@ def test_trajectory_matches_upstream(name):
         worst = max(worst, rel_err(T[0, ::st, ::st], z["top_history"][n], rise))
     assert worst < 1e-11
     assert rel_err(T, z["final"], rise) < 1e-11
+
+
+def test_zero_curvature_is_the_uncurved_assembly_times_the_volume():
+    """Curvature mode with both curvatures 0: every edge keeps its nominal
+    length, so each cell's equation is the uncurved one multiplied by
+    dz*dy*dx (crank_nicolson.pyx:409-457 against :359-378)."""
+    case = cases.case_tiny_curved()
+    ny, nx = case["shape"][1:]
+    case["curvature"] = (np.zeros((ny, nx)), np.zeros((ny, nx)))
+    curved, vol = oracle.assemble_curved(case["shape"], case["d"], case["dt"],
+                                         case["materials"], case["boundaries"], case["maps"],
+                                         *case["curvature"])
+    flat = oracle.assemble(case["shape"], case["d"], case["dt"], case["materials"],
+                           case["boundaries"], case["maps"])
+    V = case["d"][0] * case["d"][1] * case["d"][2]
+    assert np.allclose(vol, V, rtol=1e-15, atol=0)
+    fixed = flat[:, 1] == 0.0  # FIXED cells keep M = 1 unscaled
+    scaled = flat * V
+    scaled[fixed] = flat[fixed]
+    assert rel_err(curved, scaled, np.abs(scaled).max()) < 1e-15
 
 
 def test_thomas_matches_dense_and_upstream_diagnostic():
