@@ -31,7 +31,8 @@ class HS2Error(RuntimeError):
 
 # every symbol include/heatsim2_amd.h declares
 EXPORTS = ("hs2_plan_create", "hs2_plan_destroy", "hs2_plan_set_compact", "hs2_plan_set_dense",
-           "hs2_plan_expand", "hs2_plan_set_config", "hs2_plan_info", "hs2_step", "hs2_run",
+           "hs2_plan_set_dense_asym", "hs2_plan_expand", "hs2_plan_set_config", "hs2_plan_info",
+           "hs2_step", "hs2_run",
            "hs2_run_timed", "hs2_grad_work_doubles", "hs2_grad_forward", "hs2_grad_backward",
            "hs2_batch_create", "hs2_batch_destroy", "hs2_batch_layout", "hs2_batch_set_dense",
            "hs2_batch_set_options", "hs2_batch_forward", "hs2_batch_backward",
@@ -58,6 +59,8 @@ def load():
     L.hs2_plan_set_compact.restype = i32
     L.hs2_plan_set_dense.argtypes = [vp, vp, vp, vp, vp, vp]
     L.hs2_plan_set_dense.restype = i32
+    L.hs2_plan_set_dense_asym.argtypes = [vp] * 9
+    L.hs2_plan_set_dense_asym.restype = i32
     L.hs2_plan_expand.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.hs2_plan_expand.restype = i32
     L.hs2_plan_set_config.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32, i32]

@@ -40,8 +40,15 @@ def source_tables(ADI_params, t, dt, volumetric, volumetric_elements):
                 active = True
         elif kind == IMPULSE_POINT_SOURCE_JOULES:
             if t == entry[1]:
-                vtab[idx] = entry[2] / (ADI_params.volume_array * dt)
-                active = True
+                vol = ADI_params.volume_array
+                if np.ndim(vol):  # per-cell volumes (curvature mode), :310-312
+                    sel = volumetric_elements == idx
+                    if sfield is None:
+                        sfield = np.zeros(ADI_params.shape)
+                    sfield[sel] = entry[2] / (vol[sel] * dt)
+                else:
+                    vtab[idx] = entry[2] / (vol * dt)
+                    active = True
         elif kind == SPATIALLY_Z_DECAYING_TEMPORAL_IMPULSE:
             (_, impulse_time, decaydirec, offset, z_ndgrid, ddz, joulesperm2,
              characlength) = entry

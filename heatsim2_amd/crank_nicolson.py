@@ -130,12 +130,10 @@ class _FaceLaws(object):
             self._expr[key] = fn(KM, KP, dz, dy, dx, *Ts, *extra)
         return self._expr[key]
 
-    def g(self, b, axis, km, kp, kkey):
-        key = (b, axis, kkey)
-        if key in self._cache:
-            return self._cache[key]
+    def _alpha(self, expr, b, axis, km, kp):
+        """alpha of the flux alpha*(T_plus - T_minus) that expr states"""
         where = "boundary %d (%s faces)" % (b, _AXIS_NAME[axis])
-        const, terms = expression.linear_form(self.expr(b, axis), {"kmat_m": km, "kmat_p": kp})
+        const, terms = expression.linear_form(expr, {"kmat_m": km, "kmat_p": kp})
         if const != 0.0:
             raise ValueError("%s: the flux has a constant term %g; only fluxes linear in the "
                              "temperatures are supported (the upstream asserts on it)"
@@ -158,6 +156,42 @@ class _FaceLaws(object):
         if abs(alpha + beta) > 1e-12 * max(abs(alpha), abs(beta)):
             raise ValueError("%s: flux %g*T_plus + %g*T_minus is not a two-point difference"
                              % (where, alpha, beta))
+        return alpha
+
+    def alpha_in_spacings(self, b, axis, km, kp):
+        """(A, B, C) with alpha(dy, dx) = A + B/dx + C/dy for the face law
+        evaluated at other y and x spacings (curvature mode substitutes each
+        cell's mean spacings, crank_nicolson.pyx:433-434).  Probed at powers
+        of two, where the shipped laws (-k/d, -h, 0) are exact, and checked
+        at a fourth point; a law of another form is rejected."""
+        bnd = self.boundaries[b]
+        plugin, extra = bnd[0], tuple(bnd[1:])
+        fn = (plugin.qz, plugin.qy, plugin.qx)[axis]
+        KM = expression.linear_expression("kmat_m")
+        KP = expression.linear_expression("kmat_p")
+        Ts = [expression.linear_expression(n) for n in _T_ARGS[axis]]
+        dz = self.d[0]
+
+        def at(dy, dx):
+            return self._alpha(fn(KM, KP, dz, dy, dx, *Ts, *extra), b, axis, km, kp)
+
+        a1, a2, a3 = at(1.0, 1.0), at(2.0, 1.0), at(1.0, 2.0)
+        C = 2.0 * (a1 - a2)
+        B = 2.0 * (a1 - a3)
+        A = a1 - B - C
+        a4 = at(4.0, 8.0)
+        if abs(A + B / 8.0 + C / 4.0 - a4) > 1e-12 * max(abs(a1), abs(a4), 1e-300):
+            raise NotImplementedError(
+                "boundary %d (%s faces): curvature mode needs a face law of the form "
+                "a + b/dx + c/dy" % (b, _AXIS_NAME[axis]))
+        return A, B, C
+
+    def g(self, b, axis, km, kp, kkey):
+        key = (b, axis, kkey)
+        if key in self._cache:
+            return self._cache[key]
+        where = "boundary %d (%s faces)" % (b, _AXIS_NAME[axis])
+        alpha = self._alpha(self.expr(b, axis), b, axis, km, kp)
         g = -alpha / self.d[axis]
         if g < 0.0:
             raise ValueError("%s: negative face conductance %g" % (where, g))
@@ -236,6 +270,79 @@ def build_tables(d, dt, materials, boundaries, mat, bz, by, bx):
     return gtab, mtab, compute
 
 
+def curved_geometry(d, nz, curv_y, curv_x):
+    """Edge lengths and volumes of curvature mode (crank_nicolson.pyx:
+    409-424): under a top surface of curvature kappa, an edge k*dz deep is
+    (1 + k*kappa*dz) times its nominal length; a cell takes the mean of its
+    top and bottom edges, and the volume |dx_mean * dy_mean * dz|."""
+    dz, dy, dx = d
+    k = np.arange(nz, dtype=np.float64)[:, None, None]
+    cy = np.asarray(curv_y, dtype=np.float64)[None]
+    cx = np.asarray(curv_x, dtype=np.float64)[None]
+    geo = dict(dy_top=(1.0 + k * cy * dz) * dy, dy_bot=(1.0 + (k + 1.0) * cy * dz) * dy,
+               dx_top=(1.0 + k * cx * dz) * dx, dx_bot=(1.0 + (k + 1.0) * cx * dz) * dx)
+    for key, val in geo.items():
+        if not (val > 0.0).all():
+            raise ValueError("curvature mode: %s edges are not positive everywhere (a "
+                             "curvature of magnitude >= 1/depth)" % key.split("_")[0][1:])
+    geo["dy_mean"] = (geo["dy_top"] + geo["dy_bot"]) / 2.0
+    geo["dx_mean"] = (geo["dx_top"] + geo["dx_bot"]) / 2.0
+    geo["vol"] = np.abs(geo["dx_mean"] * geo["dy_mean"] * dz)
+    return geo
+
+
+def curved_coefficients(d, materials, boundaries, mat, bz, by, bx, mtab, compute, geo):
+    """Per-cell coefficients of curvature mode, per unit volume.
+
+    Each cell's balance (crank_nicolson.pyx:436-440) weighs a face by the
+    cell's own area -- dx_top*dy_top and dx_bot*dy_bot for the z faces,
+    dx_mean*dz for y, dy_mean*dz for x -- and takes the face law with its own
+    mean spacings; divided by the volume it reads
+      sum_faces G_f (T_nb - T_c),  G_f = -alpha_f * area_f / V.
+    Two cells meet a y or x face with different G, so every cell gets its own
+    minus- and plus-face values.  The neighbour-k fallback is the uncurved
+    one.  Returns M, w, gm[3], gp[3] as (nz, ny, nx) arrays."""
+    nmat = len(materials)
+    fixed, kvals, _ = _material_table(materials, 1.0)
+    fixed = np.array(fixed)
+    laws = _FaceLaws(boundaries, d)
+    comp = compute[mat].astype(bool)
+    shape = mat.shape
+    dz = d[0]
+    area = ((geo["dx_top"] * geo["dy_top"], geo["dx_bot"] * geo["dy_bot"]),
+            (geo["dx_mean"] * dz,) * 2, (geo["dy_mean"] * dz,) * 2)
+    faces = ((bz[:-1], bz[1:]), (by[:, :-1], by[:, 1:]), (bx[:, :, :-1], bx[:, :, 1:]))
+    gm, gp = [], []
+    for axis in range(3):
+        lo = np.roll(mat, 1, axis=axis)
+        hi = np.roll(mat, -1, axis=axis)
+        idx = np.arange(shape[axis]).reshape([-1 if a == axis else 1 for a in range(3)])
+        lo = np.where((idx > 0) & ~fixed[lo], lo, mat)
+        hi = np.where((idx < shape[axis] - 1) & ~fixed[hi], hi, mat)
+        out = []
+        for side, (b, ma, mb) in enumerate(((faces[axis][0], lo, mat), (faces[axis][1], mat, hi))):
+            key = (b.astype(np.int64) * nmat + ma) * nmat + mb
+            ukeys, inv = np.unique(np.where(comp, key, -1), return_inverse=True)
+            coef = np.zeros((len(ukeys), 3))
+            for t, kk in enumerate(ukeys):
+                if kk < 0:
+                    continue
+                bb, rem = divmod(int(kk), nmat * nmat)
+                m1, m2 = divmod(rem, nmat)
+                coef[t] = laws.alpha_in_spacings(bb, axis, kvals[m1], kvals[m2])
+            c = coef[inv.reshape(shape)]
+            alpha = c[..., 0] + c[..., 1] / geo["dx_mean"] + c[..., 2] / geo["dy_mean"]
+            G = np.where(comp, -alpha * area[axis][side] / geo["vol"], 0.0)
+            if (G < 0.0).any():
+                raise ValueError("curvature mode: negative %s face conductance" % _AXIS_NAME[axis])
+            out.append(G)
+        gm.append(out[0])
+        gp.append(out[1])
+    M = mtab[mat]
+    w = compute[mat].astype(np.uint8)
+    return M, w, gm, gp
+
+
 def setup(z0, y0, x0,
           dz, dy, dx,
           nz, ny, nx,
@@ -254,11 +361,15 @@ def setup(z0, y0, x0,
     """Prepare the ADI step for a grid; returns (ADI_params, ADI_steps).
 
     Same arguments as the upstream setup (crank_nicolson.pyx:128-142).
-    z0, y0, x0 are accepted and unused, as upstream.  The curvature and
-    unaligned-anisotropic modes are not provided.
+    z0, y0, x0 are accepted and unused, as upstream.  With both top-surface
+    curvature arrays (ny, nx) given, the curvature mode of
+    crank_nicolson.pyx:388-457 (per-cell areas and volumes; the volumes also
+    scale point sources).  The unaligned-anisotropic mode is not provided.
     """
-    if top_surface_y_curvatures is not None or top_surface_x_curvatures is not None:
-        raise NotImplementedError("top-surface curvature mode is not provided by heatsim2_amd")
+    curved = top_surface_y_curvatures is not None or top_surface_x_curvatures is not None
+    if curved and (top_surface_y_curvatures is None or top_surface_x_curvatures is None):
+        raise TypeError("curvature mode needs both top_surface_y_curvatures and "
+                        "top_surface_x_curvatures (the upstream indexes both)")
     if unaligned_anisotropic:
         raise NotImplementedError("unaligned_anisotropic (off-diagonal K) mode is not provided "
                                   "by heatsim2_amd")
@@ -274,6 +385,9 @@ def setup(z0, y0, x0,
     d = (float(dz), float(dy), float(dx))
     gtab, mtab, compute = build_tables(d, float(dt), materials, boundaries, mat, bz, by, bx)
     nmat, nb = len(materials), len(boundaries)
+    if curved:
+        return _setup_curved(d, float(dt), (nz, ny, nx), materials, boundaries, mat, bz, by, bx,
+                             mtab, compute, top_surface_y_curvatures, top_surface_x_curvatures)
     if gtab.size + 2 * nmat > MAX_TABLE_DOUBLES:
         raise NotImplementedError(
             "%d materials x %d boundary classes exceed the compact coefficient table (%d "
@@ -297,6 +411,42 @@ def setup(z0, y0, x0,
                                       _lib.ptr(params.d_by), _lib.ptr(params.d_bx), nmat, nb,
                                       _lib.host_ptr(gt), _lib.host_ptr(mtab),
                                       _lib.host_ptr(compute)))
+    torch.cuda.synchronize()
+    steps = [adi_step(params, (0, 1, 2), 0), adi_step(params, (2, 0, 1), 1),
+             adi_step(params, (1, 2, 0), 2)]
+    return params, steps
+
+
+def _setup_curved(d, dt, shape, materials, boundaries, mat, bz, by, bx, mtab, compute, cy, cx):
+    nz, ny, nx = shape
+    cy = np.asarray(cy, dtype=np.float64)
+    cx = np.asarray(cx, dtype=np.float64)
+    for name, c in (("top_surface_y_curvatures", cy), ("top_surface_x_curvatures", cx)):
+        if c.shape != (ny, nx):
+            raise ValueError("%s has shape %s, expected %s" % (name, c.shape, (ny, nx)))
+        if not np.isfinite(c).all():
+            raise ValueError("%s must be finite" % name)
+    geo = curved_geometry(d, nz, cy, cx)
+    M, w, gm, gp = curved_coefficients(d, materials, boundaries, mat, bz, by, bx, mtab, compute,
+                                       geo)
+    L = _lib.load()
+    dev = _lib.device()
+    params = adi_params(shape=(nz, ny, nx), volume_array=np.ascontiguousarray(geo["vol"]),
+                        d=d, dt=dt, nmat=len(materials), nb=len(boundaries), mtab=mtab,
+                        compute=compute, curvatures=(cy.copy(), cx.copy()))
+    params.d_coef = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+                     for a in [M] + gm + gp]
+    params.d_w = torch.from_numpy(np.ascontiguousarray(w)).to(dev)
+    import ctypes
+    grid = _lib.GridDesc(nz, ny, nx, d[0], d[1], d[2], dt)
+    h = ctypes.c_void_p()
+    _lib.check("hs2_plan_create", L.hs2_plan_create(ctypes.byref(grid), ctypes.byref(h)))
+    params._plan = h
+    c = params.d_coef
+    _lib.check("hs2_plan_set_dense_asym",
+               L.hs2_plan_set_dense_asym(h, _lib.ptr(c[0]), _lib.ptr(params.d_w), _lib.ptr(c[1]),
+                                         _lib.ptr(c[2]), _lib.ptr(c[3]), _lib.ptr(c[4]),
+                                         _lib.ptr(c[5]), _lib.ptr(c[6])))
     torch.cuda.synchronize()
     steps = [adi_step(params, (0, 1, 2), 0), adi_step(params, (2, 0, 1), 1),
              adi_step(params, (1, 2, 0), 2)]

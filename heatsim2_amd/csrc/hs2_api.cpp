@@ -36,7 +36,7 @@ int fail(int code, const char *fmt, ...) {
     if (e_ != hipSuccess) return fail(HS2_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-enum { MODE_NONE = 0, MODE_CLASS = 1, MODE_DENSE = 2 };
+enum { MODE_NONE = 0, MODE_CLASS = 1, MODE_DENSE = 2, MODE_ASYM = 3 };
 
 constexpr int kCheckBlocks = 256;
 constexpr int kMaxBlockThreads = 512;  // __launch_bounds__ of the strided kernel
@@ -50,6 +50,7 @@ struct hs2_plan {
   int mode = MODE_NONE;
   hs2::ClassView cv{};
   hs2::DenseView dv{};
+  hs2::AsymView av{};
   int ncls[3] = {0, 0, 0};
   uint8_t *d_cls = nullptr;     // 3n class ids (class mode)
   double *d_tab = nullptr;      // class tables
@@ -180,17 +181,24 @@ hs2::Grid grid_of(const hs2_plan *p) { return hs2::Grid{p->grid.nz, p->grid.ny, 
 
 hipError_t sweep_x(const hs2_plan *p, const double *Tn, double *Ts, const uint8_t *vmap,
                    const double *vtab, const double *sfield, hipStream_t s) {
-  return p->mode == MODE_CLASS
-             ? hs2::launch_sweep_x(p->xcfg, grid_of(p), p->cv, Tn, Ts, vmap, vtab, sfield, s)
-             : hs2::launch_sweep_x(p->xcfg, grid_of(p), p->dv, Tn, Ts, vmap, vtab, sfield, s);
+  const hs2::Grid g = grid_of(p);
+  if (p->mode == MODE_CLASS)
+    return hs2::launch_sweep_x(p->xcfg, g, p->cv, Tn, Ts, vmap, vtab, sfield, s);
+  if (p->mode == MODE_ASYM)
+    return hs2::launch_sweep_x(p->xcfg, g, p->av, Tn, Ts, vmap, vtab, sfield, s);
+  return hs2::launch_sweep_x(p->xcfg, g, p->dv, Tn, Ts, vmap, vtab, sfield, s);
 }
 
 hipError_t sweep_s(const hs2_plan *p, int axis, const double *Tn, const double *Y, double *X,
                    double *surf, hipStream_t s) {
   const hs2::SConfig c = p->scfg[axis];
-  return p->mode == MODE_CLASS
-             ? hs2::launch_sweep_strided(c, grid_of(p), p->cv, axis, Tn, Y, X, surf, p->grid.dz, s)
-             : hs2::launch_sweep_strided(c, grid_of(p), p->dv, axis, Tn, Y, X, surf, p->grid.dz, s);
+  const hs2::Grid g = grid_of(p);
+  const double dz = p->grid.dz;
+  if (p->mode == MODE_CLASS)
+    return hs2::launch_sweep_strided(c, g, p->cv, axis, Tn, Y, X, surf, dz, s);
+  if (p->mode == MODE_ASYM)
+    return hs2::launch_sweep_strided(c, g, p->av, axis, Tn, Y, X, surf, dz, s);
+  return hs2::launch_sweep_strided(c, g, p->dv, axis, Tn, Y, X, surf, dz, s);
 }
 
 int run_step(hs2_plan *p, const double *Tin, double *Tout, const uint8_t *vmap,
@@ -594,6 +602,28 @@ int hs2_plan_set_compact(hs2_plan *p, const uint8_t *d_mat, const uint8_t *d_bz,
   return HS2_OK;
 }
 
+int hs2_plan_set_dense_asym(hs2_plan *p, const double *d_M, const uint8_t *d_w,
+                            const double *d_gzm, const double *d_gym, const double *d_gxm,
+                            const double *d_gzp, const double *d_gyp, const double *d_gxp) {
+  const double *g[6] = {d_gzm, d_gym, d_gxm, d_gzp, d_gyp, d_gxp};
+  if (!p || !d_M || !d_w) return fail(HS2_E_ARG, "hs2_plan_set_dense_asym: null argument");
+  for (const double *a : g)
+    if (!a) return fail(HS2_E_ARG, "hs2_plan_set_dense_asym: null argument");
+  if (!aligned16(d_M) || !aligned16(d_w))
+    return fail(HS2_E_ARG, "hs2_plan_set_dense_asym: arrays must be 16-byte aligned");
+  for (const double *a : g)
+    if (!aligned16(a)) return fail(HS2_E_ARG, "hs2_plan_set_dense_asym: arrays must be 16-byte aligned");
+  free_coefficients(p);
+  p->av.M = d_M;
+  p->av.w = d_w;
+  for (int a = 0; a < 3; a++) {
+    p->av.g[a] = g[a];
+    p->av.gp[a] = g[3 + a];
+  }
+  p->mode = MODE_ASYM;
+  return HS2_OK;
+}
+
 int hs2_plan_set_dense(hs2_plan *p, const double *d_M, const uint8_t *d_w, const double *d_gz,
                        const double *d_gy, const double *d_gx) {
   if (!p || !d_M || !d_w || !d_gz || !d_gy || !d_gx)
@@ -652,6 +682,9 @@ int hs2_plan_expand(hs2_plan *p, double *d_M, double *d_w, double *d_gz, double 
                     double *d_gx, void *stream) {
   if (!p) return fail(HS2_E_ARG, "hs2_plan_expand: null plan");
   if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_plan_expand: coefficients not set");
+  if (p->mode == MODE_ASYM)
+    return fail(HS2_E_UNSUPPORTED, "hs2_plan_expand: the plan's coefficients are the caller's "
+                                   "(hs2_plan_set_dense_asym)");
   hipStream_t s = (hipStream_t)stream;
   if (p->mode == MODE_CLASS)
     HIP_TRY(hs2::launch_expand(grid_of(p), p->cv, d_M, d_w, d_gz, d_gy, d_gx, s));
@@ -783,6 +816,8 @@ int hs2_grad_forward(hs2_plan *p, double *d_T, int nsteps, int ckpt_every, doubl
       !aligned16(d_ckpt))
     return fail(HS2_E_ARG, "hs2_grad_forward: bad argument");
   if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_grad_forward: coefficients not set");
+  if (p->mode == MODE_ASYM)
+    return fail(HS2_E_UNSUPPORTED, "hs2_grad_forward: no adjoint for per-side face coefficients");
   if (d_surf_hist && p->grid.nz < 2)
     return fail(HS2_E_ARG, "hs2_grad_forward: the surface readout needs nz >= 2");
   std::vector<int> src;
@@ -810,6 +845,8 @@ int hs2_grad_backward(hs2_plan *p, int nsteps, int ckpt_every, const double *d_c
       nsteps < 1 || ckpt_every < 1 || (nsrc > 0 && !d_gsrc))
     return fail(HS2_E_ARG, "hs2_grad_backward: bad argument");
   if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_grad_backward: coefficients not set");
+  if (p->mode == MODE_ASYM)
+    return fail(HS2_E_UNSUPPORTED, "hs2_grad_backward: no adjoint for per-side face coefficients");
   if (p->grid.nz < 2) return fail(HS2_E_ARG, "hs2_grad_backward: the surface readout needs nz >= 2");
   if (!aligned16(d_ckpt) || !aligned16(d_work))
     return fail(HS2_E_ARG, "hs2_grad_backward: d_ckpt and d_work must be 16-byte aligned");

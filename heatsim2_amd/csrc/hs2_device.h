@@ -35,10 +35,12 @@ struct Grid {
 
 struct Entry {
   double M, w, g;
+  double gp;  // plus-face conductance as the cell sees it (views with kAsym only)
 };
 
 // ---------------------------------------------------------------- views
 struct ClassView {
+  static constexpr bool kAsym = false;  // a cell's plus face is its neighbour's minus face
   const uint8_t *__restrict__ cls[3];  // per axis 0=z, 1=y, 2=x; n each
   const double *__restrict__ tab;      // global tables, axis a at tab + off[a]
   int off[4];                          // doubles; off[3] = total length
@@ -78,6 +80,7 @@ struct DenseRaw {
 };
 
 struct DenseView {
+  static constexpr bool kAsym = false;
   const double *__restrict__ M;
   const uint8_t *__restrict__ w;
   const double *__restrict__ g[3];
@@ -101,6 +104,36 @@ struct DenseView {
   }
   __device__ __forceinline__ Entry resolve(const double *, Raw r) const {
     return Entry{r.M, (double)r.w, r.g};
+  }
+};
+
+struct AsymRaw {
+  double M, g, gp;
+  uint32_t w;
+};
+
+// Per-voxel coefficients where the two cells of a face see it differently
+// (curvature mode: each cell weighs a face by its own area and volume), so
+// each cell carries its plus-face conductance besides its minus-face one.
+struct AsymView {
+  static constexpr bool kAsym = true;
+  const double *__restrict__ M;
+  const uint8_t *__restrict__ w;
+  const double *__restrict__ g[3];   // minus faces
+  const double *__restrict__ gp[3];  // plus faces
+  using Raw = AsymRaw;
+  __host__ __device__ int lds_doubles(int) const { return 0; }
+  __device__ void load_tables(double *, int) const {}
+  __device__ __forceinline__ const double *table(const double *lds, int) const { return lds; }
+  __device__ __forceinline__ Raw fetch(int axis, int c) const {
+    return AsymRaw{M[c], g[axis][c], gp[axis][c], w[c]};
+  }
+  __device__ __forceinline__ void fetch4(int axis, int c, Raw (&r)[4]) const {
+#pragma unroll
+    for (int q = 0; q < 4; q++) r[q] = fetch(axis, c + q);
+  }
+  __device__ __forceinline__ Entry resolve(const double *, Raw r) const {
+    return Entry{r.M, (double)r.w, r.g, r.gp};
   }
 };
 

@@ -27,6 +27,16 @@
 
 namespace hs2 {
 
+// Conductance of a cell's plus face on one axis: the cell's own (views whose
+// cells see a face differently from either side) or else the minus face of
+// the next cell, which is the same face.
+template <class V>
+__device__ __forceinline__ double plus_g(const V &view, const double *tab, typename V::Raw own,
+                                         typename V::Raw next) {
+  if constexpr (V::kAsym) return view.resolve(tab, own).gp;
+  else return view.resolve(tab, next).g;
+}
+
 // ---------------------------------------------------------------- x sweep
 template <class V, int CPL, int G>
 __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *__restrict__ Tn,
@@ -69,9 +79,9 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
     Tzp[q] = Tn[bzp + i];
     rx[q] = view.fetch(2, base + i);
     ry[q] = view.fetch(1, base + i);
-    ryp[q] = view.fetch(1, byp + i);
+    ryp[q] = V::kAsym ? ry[q] : view.fetch(1, byp + i);
     rz[q] = view.fetch(0, base + i);
-    rzp[q] = view.fetch(0, bzp + i);
+    rzp[q] = V::kAsym ? rz[q] : view.fetch(0, bzp + i);
   }
   double S[CPL];
 #pragma unroll
@@ -100,11 +110,11 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
     const bool valid = lvalid && i < nx;
     const double M = ex[q].M, w = ex[q].w;
     const double gxm = i > 0 ? ex[q].g : 0.0;
-    const double gxp = i < nx - 1 ? ex[q + 1].g : 0.0;
+    const double gxp = i < nx - 1 ? (V::kAsym ? ex[q].gp : ex[q + 1].g) : 0.0;
     const double gym = j > 0 ? view.resolve(taby, ry[q]).g : 0.0;
-    const double gyp = j < ny - 1 ? view.resolve(taby, ryp[q]).g : 0.0;
+    const double gyp = j < ny - 1 ? plus_g(view, taby, ry[q], ryp[q]) : 0.0;
     const double gzm = k > 0 ? view.resolve(tabz, rz[q]).g : 0.0;
-    const double gzp = k < nz - 1 ? view.resolve(tabz, rzp[q]).g : 0.0;
+    const double gzp = k < nz - 1 ? plus_g(view, tabz, rz[q], rzp[q]) : 0.0;
     const double Tc = T[q];
     const double Txm = q > 0 ? T[q - 1] : TL;
     const double Txp = q < CPL - 1 ? T[q + 1] : TR;
@@ -273,9 +283,17 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
         view.fetch4(2, b0 + x0, rx[b]);
         rxn[b] = view.fetch(2, b0 + xr);
         view.fetch4(1, b0 + x0, ry[b]);
-        view.fetch4(1, lbase[lb][2] + x0, ryp[b]);
         view.fetch4(0, b0 + x0, rz[b]);
-        view.fetch4(0, lbase[lb][4] + x0, rzp[b]);
+        if constexpr (V::kAsym) {
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            ryp[b][u] = ry[b][u];
+            rzp[b][u] = rz[b][u];
+          }
+        } else {
+          view.fetch4(1, lbase[lb][2] + x0, ryp[b]);
+          view.fetch4(0, lbase[lb][4] + x0, rzp[b]);
+        }
         }
       }
 #pragma unroll
@@ -293,11 +311,11 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           const double T0 = T[b][q];
           const double tl = q > 0 ? T[b][q - 1] : TL[b];
           const double tr = q < 3 ? T[b][q + 1] : TR[b];
-          const double gxm = ex[q].g, gxp = ex[q + 1].g;
+          const double gxm = ex[q].g, gxp = V::kAsym ? ex[q].gp : ex[q + 1].g;
           const double gym = view.resolve(taby, ry[b][q]).g;
-          const double gyp = view.resolve(taby, ryp[b][q]).g;
+          const double gyp = plus_g(view, taby, ry[b][q], ryp[b][q]);
           const double gzm = view.resolve(tabz, rz[b][q]).g;
-          const double gzp = view.resolve(tabz, rzp[b][q]).g;
+          const double gzp = plus_g(view, tabz, rz[b][q], rzp[b][q]);
           double expl = 0.5 * (gxm * (tl - T0) + gxp * (tr - T0)) + gym * (Tym[b][q] - T0) +
                         gyp * (Typ[b][q] - T0) + gzm * (Tzm[b][q] - T0) + gzp * (Tzp[b][q] - T0);
           if (SRC) {
@@ -345,11 +363,11 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
         Tzm[b] = Tn[lbase[lb][3] + x];
         Tzp[b] = Tn[lbase[lb][4] + x];
         rx[b] = view.fetch(2, b0 + x);
-        rxp[b] = view.fetch(2, b0 + xp);
+        rxp[b] = V::kAsym ? rx[b] : view.fetch(2, b0 + xp);
         ry[b] = view.fetch(1, b0 + x);
-        ryp[b] = view.fetch(1, lbase[lb][2] + x);
+        ryp[b] = V::kAsym ? ry[b] : view.fetch(1, lbase[lb][2] + x);
         rz[b] = view.fetch(0, b0 + x);
-        rzp[b] = view.fetch(0, lbase[lb][4] + x);
+        rzp[b] = V::kAsym ? rz[b] : view.fetch(0, lbase[lb][4] + x);
       }
 #pragma unroll
       for (int b = 0; b < U; b++) {
@@ -357,11 +375,11 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
         const int x = ii[b];
         const Entry e = view.resolve(tabx, rx[b]);
         const double gxm = e.g;
-        const double gxp = view.resolve(tabx, rxp[b]).g;
+        const double gxp = plus_g(view, tabx, rx[b], rxp[b]);
         const double gym = view.resolve(taby, ry[b]).g;
-        const double gyp = view.resolve(taby, ryp[b]).g;
+        const double gyp = plus_g(view, taby, ry[b], ryp[b]);
         const double gzm = view.resolve(tabz, rz[b]).g;
-        const double gzp = view.resolve(tabz, rzp[b]).g;
+        const double gzp = plus_g(view, tabz, rz[b], rzp[b]);
         const double T0 = Tc[b];
         double expl = 0.5 * (gxm * (Txm[b] - T0) + gxp * (Txp[b] - T0)) + gym * (Tym[b] - T0) +
                       gyp * (Typ[b] - T0) + gzm * (Tzm[b] - T0) + gzp * (Tzp[b] - T0);
@@ -423,13 +441,13 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
       else en.g = g_after;
       if constexpr (FULL) {  // every lane's cells lie inside its (existing) line
         const double gxm = (q == 0 && lig == 0) ? 0.0 : ec.g;
-        const double gxp = (q == SEG - 1 && lig == G - 1) ? 0.0 : en.g;
+        const double gxp = (q == SEG - 1 && lig == G - 1) ? 0.0 : V::kAsym ? ec.gp : en.g;
         const double am = -0.5 * ec.w * gxm, cp = -0.5 * ec.w * gxp;
         seg.row(q, am, ec.M - am - cp, cp, seg.rp[q]);
       } else {
         const bool valid = lvalid && x < nx;
         const double gxm = x > 0 ? ec.g : 0.0;
-        const double gxp = x < nx - 1 ? en.g : 0.0;
+        const double gxp = x < nx - 1 ? (V::kAsym ? ec.gp : en.g) : 0.0;
         const double am = -0.5 * ec.w * gxm, cp = -0.5 * ec.w * gxp;
         const double rq = valid ? seg.rp[q] : 0.0;
         seg.row(q, valid ? am : 0.0, valid ? ec.M - am - cp : 1.0, valid ? cp : 0.0, rq);
@@ -454,7 +472,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
       const int x = i0 + q;
       const bool valid = lvalid && x < nx;
       const double gxm = x > 0 ? ex[q].g : 0.0;
-      const double gxp = x < nx - 1 ? ex[q + 1].g : 0.0;
+      const double gxp = x < nx - 1 ? (V::kAsym ? ex[q].gp : ex[q + 1].g) : 0.0;
       const double am = -0.5 * ex[q].w * gxm, cp = -0.5 * ex[q].w * gxp;
       seg.row(q, valid ? am : 0.0, valid ? ex[q].M - am - cp : 1.0, valid ? cp : 0.0,
               valid ? rh[q] : 0.0);
@@ -573,7 +591,8 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
     const Entry ep = view.resolve(lds, raw[q + 1]);
     // FULL: only the line's first and last cells have an outer face
     const double gm = (FULL ? (q > 0 || p > 0) : t >= 1) ? en.g : 0.0;
-    const double gp = (FULL ? (q < SEG - 1 || t0 + SEG < L) : t + 1 <= L - 1) ? ep.g : 0.0;
+    const double gp =
+        (FULL ? (q < SEG - 1 || t0 + SEG < L) : t + 1 <= L - 1) ? (V::kAsym ? en.gp : ep.g) : 0.0;
     const double Tc = T[q + 1];
     const double M = en.M, w = en.w;
     const double rhs = M * Yv[q] - 0.5 * w * (gm * (T[q] - Tc) + gp * (T[q + 2] - Tc));
@@ -759,6 +778,11 @@ hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const DenseView &v, const 
                           const double *sfield, hipStream_t s) {
   return launch_x_v(cfg, g, v, Tn, Ts, vmap, vtab, sfield, s);
 }
+hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const AsymView &v, const double *Tn,
+                          double *Ts, const uint8_t *vmap, const double *vtab,
+                          const double *sfield, hipStream_t s) {
+  return launch_x_v(cfg, g, v, Tn, Ts, vmap, vtab, sfield, s);
+}
 
 template <class V>
 static size_t strided_lds_bytes_t(const V &v, int axis, int P, int CW) {
@@ -819,6 +843,11 @@ hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const ClassView &v, 
   return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, s);
 }
 hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const DenseView &v, int axis,
+                                const double *Tn, const double *Y, double *X, double *surf,
+                                double dz, hipStream_t s) {
+  return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, s);
+}
+hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const AsymView &v, int axis,
                                 const double *Tn, const double *Y, double *X, double *surf,
                                 double dz, hipStream_t s) {
   return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, s);

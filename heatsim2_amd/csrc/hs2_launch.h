@@ -22,6 +22,9 @@ hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const ClassView &v, const 
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const DenseView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
                           const double *sfield, hipStream_t s);
+hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const AsymView &v, const double *Tn,
+                          double *Ts, const uint8_t *vmap, const double *vtab,
+                          const double *sfield, hipStream_t s);
 
 size_t strided_lds_bytes(const ClassView &v, int axis, int P, int CW);
 size_t strided_lds_bytes(const DenseView &v, int axis, int P, int CW);
@@ -30,6 +33,9 @@ hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const ClassView &v, 
                                 const double *Tn, const double *Y, double *X, double *surf,
                                 double dz, hipStream_t s);
 hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const DenseView &v, int axis,
+                                const double *Tn, const double *Y, double *X, double *surf,
+                                double dz, hipStream_t s);
+hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const AsymView &v, int axis,
                                 const double *Tn, const double *Y, double *X, double *surf,
                                 double dz, hipStream_t s);
 

@@ -83,6 +83,18 @@ int hs2_plan_set_compact(hs2_plan *plan, const uint8_t *d_mat, const uint8_t *d_
 int hs2_plan_set_dense(hs2_plan *plan, const double *d_M, const uint8_t *d_w, const double *d_gz,
                        const double *d_gy, const double *d_gx);
 
+/* Per-voxel coefficients whose faces are seen differently from the two
+ * sides: as hs2_plan_set_dense, plus each cell's own plus-face conductances
+ * d_gzp, d_gyp, d_gxp (the face towards k+1, j+1, i+1), which need not equal
+ * the next cell's minus-face values.  This is the upstream's top-surface
+ * curvature mode, where each cell weighs its faces by its own areas and
+ * volume (crank_nicolson.pyx:388-457, replacing the per-cell add_equation
+ * assembly of alternatingdirection_c.c:102-199 for that mode).  Forward
+ * steps only: the gradient entry points reject such a plan. */
+int hs2_plan_set_dense_asym(hs2_plan *plan, const double *d_M, const uint8_t *d_w,
+                            const double *d_gzm, const double *d_gym, const double *d_gxm,
+                            const double *d_gzp, const double *d_gyp, const double *d_gxp);
+
 /* Write the plan's per-cell coefficients as float64 arrays (n each; any
  * pointer may be NULL): M, w, and the minus-face conductances per axis --
  * the numbers the upstream keeps in Amat/Bmat

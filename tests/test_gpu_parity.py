@@ -162,6 +162,66 @@ def test_random_problems_match_oracle(hs, shape):
         assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, n
 
 
+@pytest.mark.parametrize("on_device", [False, True])
+def test_curved_case_matches_upstream(hs, on_device):
+    """Curvature mode against the upstream's own trajectory (tiny_curved:
+    seeded curvatures of both signs, thin z/y/x faces, FIXED cells, impulse,
+    stepped and volume-scaled point sources)."""
+    case = load_golden("tiny_curved")
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    assert _lib_mode(P) == 3
+    ref = case["fixture"]["traj"]
+    rise = np.abs(ref - case["T0"]).max()
+    T = torch.from_numpy(case["T0"]).cuda() if on_device else case["T0"].copy()
+    for n in range(case["nsteps"]):
+        T = hs.run_adi_steps(P, S, case["t0"] + n * case["dt"], case["dt"], T, args["vmap"],
+                             args["volumetric"])
+        got = T.cpu().numpy() if on_device else T
+        assert _rel(got, ref[n], rise) < TOL, n
+
+
+def _lib_mode(P):
+    from heatsim2_amd import _lib
+    return _lib.plan_info(P.plan)["mode"]
+
+
+def _curved_case(shape, seed):
+    case = _random_case(shape, seed)
+    rng = np.random.default_rng(seed + 1)
+    nz, ny, nx = shape
+    kmax = 0.5 / (nz * case["d"][0])  # edges change by at most half over the depth
+    case["curvature"] = (rng.uniform(-kmax, kmax, (ny, nx)), rng.uniform(-kmax, kmax, (ny, nx)))
+    v = case["maps"][4]
+    v[rng.random(shape) < 0.05] = 3
+    case["volumetric"] = list(case["volumetric"]) + [(3, 2 * case["dt"], 0.05)]
+    case["nsteps"] = 5
+    return case
+
+
+@pytest.mark.parametrize("shape", [(5, 16, 64), (7, 12, 256), (9, 20, 33), (40, 3, 5),
+                                   (3, 130, 17)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_curved_random_problems_match_oracle(hs, shape):
+    """Curvature mode on random problems whose lines take the full-line and
+    the masked x kernels and every strided segment length, against the
+    oracle's curvature assembly (pinned to the upstream) and its
+    reference-structured step."""
+    case = _curved_case(shape, seed=31 + sum(shape))
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    ref = oracle.run_ref(case)
+    rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
+    T = torch.from_numpy(case["T0"]).cuda()
+    for n in range(case["nsteps"]):
+        T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"], args["volumetric"])
+        assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, n
+    T2, _ = hs.run_adi_steps_many(P, 0.0, case["dt"], case["nsteps"],
+                                  torch.from_numpy(case["T0"]).cuda(), args["vmap"],
+                                  args["volumetric"])
+    assert torch.equal(T, T2)
+
+
 def test_run_many_equals_stepwise(hs):
     case = load_golden("tiny_mixed")
     args = to_package_args(hs, case)
@@ -214,10 +274,26 @@ def test_every_launch_shape_matches_oracle(hs, shape):
     without source terms.  From (4, 8, 256) on, the grids fill the x lanes
     exactly (the unmasked x path where the tiles also cover the planes and
     rows; ragged tiles elsewhere, e.g. 13 planes)."""
-    from heatsim2_amd import _lib, autotune
     case = _random_case(shape, seed=7 + sum(shape))
     # the stepped source lasts through t = 3 ms: steps 4 and 5 have none
     case["nsteps"] = 6
+    _every_launch_shape(hs, case)
+
+
+@pytest.mark.parametrize("shape", [(20, 48, 200), (4, 8, 256), (6, 4, 64), (2, 16, 32),
+                                   (13, 24, 128)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_every_launch_shape_curved(hs, shape):
+    """The same over the curvature mode's per-side coefficients (each kernel
+    variant takes a cell's plus faces from the cell itself)."""
+    case = _curved_case(shape, seed=11 + sum(shape))
+    case["nsteps"] = 6
+    _every_launch_shape(hs, case)
+
+
+def _every_launch_shape(hs, case):
+    from heatsim2_amd import _lib, autotune
+    shape = case["shape"]
     args = to_package_args(hs, case)
     P, S = hs.setup(*args["setup"])
     ref = oracle.run_ref(case)

@@ -113,9 +113,50 @@ def test_setup_rejects_wrong_conductivity_type():
 def test_setup_modes_not_provided():
     a = _setup_args()
     with pytest.raises(NotImplementedError):
-        hs.setup(*a, top_surface_y_curvatures=np.zeros(3))
-    with pytest.raises(NotImplementedError):
         hs.setup(*a, unaligned_anisotropic=True)
+
+
+def test_curvature_setup_arguments():
+    """Both curvature arrays or none (the upstream indexes both,
+    crank_nicolson.pyx:399); (ny, nx) shapes; edges stay positive."""
+    a = _setup_args()
+    ny, nx = a[7], a[8]
+    with pytest.raises(TypeError):
+        hs.setup(*a, top_surface_y_curvatures=np.zeros((ny, nx)))
+    with pytest.raises(ValueError, match="shape"):
+        hs.setup(*a, top_surface_y_curvatures=np.zeros((ny, nx)),
+                 top_surface_x_curvatures=np.zeros((ny, nx + 1)))
+    big = np.full((ny, nx), -1.0 / (a[3] * a[6]))  # edges shrink to zero at the bottom
+    with pytest.raises(ValueError, match="not positive"):
+        hs.setup(*a, top_surface_y_curvatures=big, top_surface_x_curvatures=np.zeros((ny, nx)))
+
+
+def test_curved_coefficients_match_oracle():
+    """The curvature-mode coefficients setup() uploads (per unit volume, each
+    cell's own minus and plus faces) equal the oracle's volume-multiplied
+    assembly divided by the volume; the oracle is pinned to the upstream's
+    assembled operators for this case (test_oracle)."""
+    from heatsim2_amd import crank_nicolson as cn
+    case = load_golden("tiny_curved")
+    a = list(to_package_args(hs, case)["setup"])
+    d = tuple(case["d"])
+    m, bz, by, bx = case["maps"][:4]
+    nz = case["shape"][0]
+    gtab, mtab, compute = cn.build_tables(d, case["dt"], a[10], a[11], m, bz, by, bx)
+    geo = cn.curved_geometry(d, nz, *case["curvature"])
+    M, w, gm, gp = cn.curved_coefficients(d, a[10], a[11], m, bz, by, bx, mtab, compute, geo)
+    ref, vol = oracle.assemble_curved(case["shape"], d, case["dt"], case["materials"],
+                                      case["boundaries"], case["maps"], *case["curvature"])
+    assert np.allclose(geo["vol"].ravel(), vol.ravel(), rtol=1e-15, atol=0)
+    comp = ref[:, 1] != 0.0
+    V = np.where(comp, vol.ravel(), 1.0)
+    got = np.stack([M.ravel(), w.ravel(), gm[0].ravel(), gp[0].ravel(), gm[1].ravel(),
+                    gp[1].ravel(), gm[2].ravel(), gp[2].ravel()], axis=-1)
+    want = ref / V[:, None]
+    want[:, 1] = comp
+    assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-300)) < 1e-14
+    # the case has cells whose y / x faces differ from the two sides
+    assert np.abs(gp[2][:, :, :-1] - gm[2][:, :, 1:]).max() > 1e-3 * gm[2].max()
 
 
 def test_grid_builders_closed_forms():
