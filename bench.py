@@ -12,9 +12,11 @@ A step is one full Douglas ADI step (x, y, z substeps) over all cells.
 `value` = cells x K / wall time of K steps run by hs2_run with the field and
 maps resident in HBM, bracketed by torch.cuda.synchronize().
 
-roofline: the dominant sweep kernel's algorithmic bytes per launch (x: 16 B
-per cell, y and z: 24 B per cell -- the FP64 field passes of the delta-form
-step, 64 B per cell-step in all) over its mean launch duration, from HIP
+roofline: the dominant sweep kernel's algorithmic bytes per launch (the FP64
+field passes: x 16 B per cell, z 24 B; y 24 B, or 16 B when the x sweep
+hands it its right-hand side, as on this workload -- 56 B per cell-step in
+all, against the canonical 64 B of the delta-form step) over its mean
+launch duration, from HIP
 events recorded on the launch stream around every substep kernel of K more
 steps run right after the timed region (hs2_run_timed; each event pair adds a
 few microseconds of its own, so the achieved rate is slightly understated);
@@ -40,7 +42,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = (16.0, 24.0, 24.0)  # x, y, z sweep algorithmic bytes
+BYTES_PER_CELL = (16.0, 24.0, 24.0)  # x, y, z sweep algorithmic bytes (y: 16 from x's R_y)
+CANONICAL_STEP_BYTES = 64.0  # the delta-form step's field passes, the metric's yardstick
 TRAFFIC_SUMMARY = "hbm_traffic_two_layer_256.txt"
 METRIC = "FP64 ADI cell-steps/s @256³ + batched fwd+adjoint gradients/s vs plain"
 
@@ -336,12 +339,15 @@ def main():
     torch.cuda.synchronize()
     kms = list(kms)
     names = ("sweep_x", "sweep_y", "sweep_z")
-    gbs = [BYTES_PER_CELL[i] * n / (kms[i] * 1e-3) / 1e9 for i in range(3)]
+    bpc = list(BYTES_PER_CELL)
+    if tuned["y_from_x"]:
+        bpc[1] = 16.0  # y reads R_y = M T* - (Ly/2) T^n and writes T**
+    gbs = [bpc[i] * n / (kms[i] * 1e-3) / 1e9 for i in range(3)]
     dom = int(np.argmax(kms))
     for i in range(3):
         log("  %s: %.4f ms/launch, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
             % (names[i], kms[i], gbs[i], 100 * gbs[i] / HBM_PEAK_GBS))
-    step_gbs = 64.0 * value / 1e9
+    step_gbs = CANONICAL_STEP_BYTES * value / 1e9
     # measured HBM bytes per launch: committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # summary of this workload (copy-calibrated), when it is for this size
     from heatsim2_amd.rocprof_summary import read_traffic
@@ -352,9 +358,11 @@ def main():
     roofline = dict(bound="hbm", kernel=names[dom], achieved=round(gbs[dom], 1),
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs[dom] / HBM_PEAK_GBS, 4),
                     traffic=traffic, traffic_source=TRAFFIC_SUMMARY if traffic else None,
-                    bytes_per_launch=BYTES_PER_CELL[dom] * n,
+                    bytes_per_launch=bpc[dom] * n,
                     kernel_ms=[round(x, 5) for x in kms],
-                    step_achieved=round(step_gbs, 1), step_frac=round(step_gbs / HBM_PEAK_GBS, 4))
+                    step_achieved=round(step_gbs, 1), step_frac=round(step_gbs / HBM_PEAK_GBS, 4),
+                    step_bytes_per_cell=sum(bpc),
+                    step_frac_own_bytes=round(sum(bpc) * value / 1e9 / HBM_PEAK_GBS, 4))
 
     out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": K,
            "warmup": args.warmup, "ms_per_step": 1e3 * wall / K, "higher_is_better": True,
@@ -365,7 +373,7 @@ def main():
                       "cells": n, "parallelism": "single GPU",
                       "launch": {k: tuned[k] for k in ("x_cpl", "x_lanes", "x_lines",
                                                        "x_lines_y", "y_seg", "y_lines", "z_seg",
-                                                       "z_lines", "mode")}},
+                                                       "z_lines", "mode", "y_from_x")}},
            "roofline": roofline}
     del T, hist
     P.release()

@@ -144,8 +144,8 @@ def surface_temperature_device(T, dz):
 
 
 def plan_info(plan):
-    info = (ctypes.c_int * 12)()
+    info = (ctypes.c_int * 13)()
     check("hs2_plan_info", load().hs2_plan_info(plan, info))
     keys = ("mode", "classes_z", "classes_y", "classes_x", "x_cpl", "x_lanes", "y_seg",
-            "y_lines", "z_seg", "z_lines", "x_lines", "x_lines_y")
+            "y_lines", "z_seg", "z_lines", "x_lines", "x_lines_y", "y_from_x")
     return dict(zip(keys, list(info)))

@@ -180,31 +180,38 @@ bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 hs2::Grid grid_of(const hs2_plan *p) { return hs2::Grid{p->grid.nz, p->grid.ny, p->grid.nx}; }
 
 hipError_t sweep_x(const hs2_plan *p, const double *Tn, double *Ts, const uint8_t *vmap,
-                   const double *vtab, const double *sfield, hipStream_t s) {
+                   const double *vtab, const double *sfield, hipStream_t s, bool ry = false) {
   const hs2::Grid g = grid_of(p);
   if (p->mode == MODE_CLASS)
-    return hs2::launch_sweep_x(p->xcfg, g, p->cv, Tn, Ts, vmap, vtab, sfield, s);
+    return hs2::launch_sweep_x(p->xcfg, g, p->cv, Tn, Ts, vmap, vtab, sfield, s, ry);
   if (p->mode == MODE_ASYM)
     return hs2::launch_sweep_x(p->xcfg, g, p->av, Tn, Ts, vmap, vtab, sfield, s);
   return hs2::launch_sweep_x(p->xcfg, g, p->dv, Tn, Ts, vmap, vtab, sfield, s);
 }
 
 hipError_t sweep_s(const hs2_plan *p, int axis, const double *Tn, const double *Y, double *X,
-                   double *surf, hipStream_t s) {
+                   double *surf, hipStream_t s, bool ry = false) {
   const hs2::SConfig c = p->scfg[axis];
   const hs2::Grid g = grid_of(p);
   const double dz = p->grid.dz;
   if (p->mode == MODE_CLASS)
-    return hs2::launch_sweep_strided(c, g, p->cv, axis, Tn, Y, X, surf, dz, s);
+    return hs2::launch_sweep_strided(c, g, p->cv, axis, Tn, Y, X, surf, dz, s, ry);
   if (p->mode == MODE_ASYM)
     return hs2::launch_sweep_strided(c, g, p->av, axis, Tn, Y, X, surf, dz, s);
   return hs2::launch_sweep_strided(c, g, p->dv, axis, Tn, Y, X, surf, dz, s);
 }
 
+// the x sweep hands the y sweep its right-hand side instead of T* (one field
+// read less) where the plan and the step allow it
+bool use_ry(const hs2_plan *p, bool src) {
+  return p->mode == MODE_CLASS && !src && hs2::x_ry_ok(p->xcfg, grid_of(p));
+}
+
 int run_step(hs2_plan *p, const double *Tin, double *Tout, const uint8_t *vmap,
              const double *d_vtab, const double *sfield, double *surf, hipStream_t s) {
-  HIP_TRY(sweep_x(p, Tin, p->d_ws, vmap, d_vtab, sfield, s));
-  HIP_TRY(sweep_s(p, 1, Tin, p->d_ws, p->d_ws, nullptr, s));
+  const bool ry = use_ry(p, vmap || sfield);
+  HIP_TRY(sweep_x(p, Tin, p->d_ws, vmap, d_vtab, sfield, s, ry));
+  HIP_TRY(sweep_s(p, 1, Tin, p->d_ws, p->d_ws, nullptr, s, ry));
   HIP_TRY(sweep_s(p, 0, Tin, p->d_ws, Tout, surf, s));
   return HS2_OK;
 }
@@ -655,6 +662,7 @@ int hs2_plan_info(const hs2_plan *p, int *info) {
   info[8] = p->scfg[0].seg;
   info[9] = p->scfg[0].cw;
   info[11] = p->xcfg.ty;
+  info[12] = use_ry(p, false) ? 1 : 0;
   return HS2_OK;
 }
 
@@ -776,12 +784,13 @@ int hs2_run_timed(hs2_plan *p, double *d_T, int nsteps, double *kernel_ms, void 
       break;
     }
   }
+  const bool ry = use_ry(p, false);
   for (int t = 0; t < nsteps && rc == HS2_OK; t++) {
     hipEvent_t *e = &ev[(size_t)t * 4];
     hipError_t err = hipEventRecord(e[0], s);
-    if (err == hipSuccess) err = sweep_x(p, d_T, p->d_ws, nullptr, nullptr, nullptr, s);
+    if (err == hipSuccess) err = sweep_x(p, d_T, p->d_ws, nullptr, nullptr, nullptr, s, ry);
     if (err == hipSuccess) err = hipEventRecord(e[1], s);
-    if (err == hipSuccess) err = sweep_s(p, 1, d_T, p->d_ws, p->d_ws, nullptr, s);
+    if (err == hipSuccess) err = sweep_s(p, 1, d_T, p->d_ws, p->d_ws, nullptr, s, ry);
     if (err == hipSuccess) err = hipEventRecord(e[2], s);
     if (err == hipSuccess) err = sweep_s(p, 0, d_T, p->d_ws, d_T, nullptr, s);
     if (err == hipSuccess) err = hipEventRecord(e[3], s);

@@ -140,7 +140,8 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
 // SEG consecutive cells per line, recomputes the x-axis coefficients from
 // the class bytes and solves like k_sweep_x.  LPB lines per block.  SRC adds
 // the volumetric source terms (only on the rare steps that carry one).
-template <class V, int SEG, int G, int LPB, bool SRC, bool FULL = false, bool BUF = false>
+template <class V, int SEG, int G, int LPB, bool SRC, bool FULL = false, bool BUF = false,
+          bool RY = false>
 __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__restrict__ Tn,
                                               double *__restrict__ Ts,
                                               const uint8_t *__restrict__ vmap,
@@ -210,19 +211,26 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   const double *tabz = view.table(lds, 0), *taby = view.table(lds, 1), *tabx = view.table(lds, 2);
 
   // ---- phase 1: right-hand sides, lanes along x
+  // quads: 4 consecutive cells per thread, vector loads; quad e = tid + NT*u
+  // of the block's LPB x nx/4 quads, walked incrementally as (l, q).  FQ
+  // (full lines, nx = SEG*G): the quads per line and the trip count are
+  // compile-time and every quad exists, so no bounds checks.
+  constexpr bool FQ = FULL && SEG % 4 == 0;
+  constexpr int UQ = FQ && SEG < 8 ? 1 : 2;
+  // RY: besides the right-hand side, keep 0.5 w (Ly T^n) and the x-class word
+  // of every quad until the store, which then writes y's right-hand side
+  // R_y = M T* - 0.5 w (Ly T^n) in place of T*, so that the y sweep needs no
+  // T^n.  The batches are unrolled to keep those in registers.
+  constexpr int NIT = FQ ? SEG / (4 * UQ) : 1;
+  static_assert(!RY || (FQ && BUF), "RY: full-line class-map kernel");
+  [[maybe_unused]] double hly[RY ? NIT : 1][UQ][4];
+  [[maybe_unused]] uint32_t rxw[RY ? NIT : 1][UQ];
   if ((nx & 3) == 0) {
-    // quads: 4 consecutive cells per thread, vector loads; quad e = tid + NT*u
-    // of the block's LPB x nx/4 quads, walked incrementally as (l, q).  FQ
-    // (full lines, nx = SEG*G): the quads per line and the trip count are
-    // compile-time and every quad exists, so no bounds checks.
-    constexpr bool FQ = FULL && SEG % 4 == 0;
-    constexpr int UQ = FQ && SEG < 8 ? 1 : 2;
     const int qpl = FQ ? SEG * G / 4 : nx >> 2;
     const int total = LPB * qpl;
     int l = tid / qpl, qx = tid - (tid / qpl) * qpl;
     const int dl = NT / qpl, dq = NT - (NT / qpl) * qpl;
-#pragma unroll 1
-    for (int e0 = 0; e0 < total; e0 += UQ * NT) {
+    auto batch = [&](const int it, const int e0) {
       int li[UQ], xq[UQ];
 #pragma unroll
       for (int b = 0; b < UQ; b++) {
@@ -260,7 +268,9 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           }
           TL[b] = buf_load_d(bT, (unsigned)(b0 + xl) * 8u);
           TR[b] = buf_load_d(bT, (unsigned)(b0 + xr) * 8u);
-          unpack4(buf_load_u32(bC[2], o[0]), rx[b]);
+          const uint32_t wx = buf_load_u32(bC[2], o[0]);
+          unpack4(wx, rx[b]);
+          if constexpr (RY) rxw[it][b] = wx;
           rxn[b] = buf_load_u8(bC[2], (unsigned)(b0 + xr));
           unpack4(buf_load_u32(bC[1], o[0]), ry[b]);
           unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
@@ -316,8 +326,16 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           const double gyp = plus_g(view, taby, ry[b][q], ryp[b][q]);
           const double gzm = view.resolve(tabz, rz[b][q]).g;
           const double gzp = plus_g(view, tabz, rz[b][q], rzp[b][q]);
-          double expl = 0.5 * (gxm * (tl - T0) + gxp * (tr - T0)) + gym * (Tym[b][q] - T0) +
-                        gyp * (Typ[b][q] - T0) + gzm * (Tzm[b][q] - T0) + gzp * (Tzp[b][q] - T0);
+          double expl;
+          if constexpr (RY) {
+            const double ly = gym * (Tym[b][q] - T0) + gyp * (Typ[b][q] - T0);
+            expl = 0.5 * (gxm * (tl - T0) + gxp * (tr - T0)) + ly + gzm * (Tzm[b][q] - T0) +
+                   gzp * (Tzp[b][q] - T0);
+            hly[it][b][q] = 0.5 * ex[q].w * ly;
+          } else {
+            expl = 0.5 * (gxm * (tl - T0) + gxp * (tr - T0)) + gym * (Tym[b][q] - T0) +
+                   gyp * (Typ[b][q] - T0) + gzm * (Tzm[b][q] - T0) + gzp * (Tzp[b][q] - T0);
+          }
           if (SRC) {
             if (vmap) expl += vtab[vmap[lbase[lb][0] + x]];
             if (sfield) expl += sfield[lbase[lb][0] + x];
@@ -330,6 +348,13 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           for (int q = 0; q < 4; q++) dst[q] = out[q];
         }
       }
+    };
+    if constexpr (RY) {
+#pragma unroll
+      for (int it = 0; it < NIT; it++) batch(it, it * UQ * NT);
+    } else {
+#pragma unroll 1
+      for (int e0 = 0; e0 < total; e0 += UQ * NT) batch(0, e0);
     }
   } else {
     // general nx: one cell per thread per item, U items per batch
@@ -489,7 +514,26 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
     }
   }
   __syncthreads();
-  if ((nx & 3) == 0) {
+  if constexpr (RY) {
+    constexpr int qpl = SEG * G / 4;
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+#pragma unroll
+      for (int b = 0; b < UQ; b++) {
+        const int e = it * UQ * NT + b * NT + tid;
+        const int l2 = e / qpl, x0 = (e % qpl) << 2;
+        const double *src = rbuf + l2 * RS + x0 + (x0 >> 4);
+        uint32_t r4[4];
+        unpack4(rxw[it][b], r4);
+        double o4[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) o4[q] = view.resolve(tabx, r4[q]).M * src[q] - hly[it][b][q];
+        const unsigned o = (unsigned)(lbase[l2][0] + x0) * 8u;
+        buf_store_d2(make_double2(o4[0], o4[1]), bS, o);
+        buf_store_d2(make_double2(o4[2], o4[3]), bS, o + 16u);
+      }
+    }
+  } else if ((nx & 3) == 0) {
     const int qpl = nx >> 2;
     for (int e = tid; e < LPB * qpl; e += NT) {
       const int l2 = e / qpl, x0 = (e - l2 * qpl) << 2;
@@ -514,7 +558,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   }
 }
 
-template <class V, int SEG, int G, int LPB, bool SRC, bool FULL>
+template <class V, int SEG, int G, int LPB, bool SRC, bool FULL, bool RY = false>
 __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
                                                     const double *__restrict__ Tn,
                                                     double *__restrict__ Ts,
@@ -522,8 +566,8 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
                                                     const double *__restrict__ vtab_g,
                                                     const double *__restrict__ sfield, int ty) {
   constexpr bool BUF = FULL && SEG % 4 == 0 && std::is_same<V, ClassView>::value;
-  sweep_x2_body<V, SEG, G, LPB, SRC, FULL, BUF>(g, view, Tn, Ts, vmap, vtab_g, sfield,
-                                                xcd_remap(blockIdx.x, gridDim.x), ty);
+  sweep_x2_body<V, SEG, G, LPB, SRC, FULL, BUF, RY>(g, view, Tn, Ts, vmap, vtab_g, sfield,
+                                                    xcd_remap(blockIdx.x, gridDim.x), ty);
 }
 
 // ---------------------------------------------------------- y / z sweeps
@@ -531,7 +575,7 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
 // line stride nx*ny.  Solves (M - La/2) X = M Y - (La/2) T^n along each line.
 // X may alias Y or T^n: a block owns whole lines, and all of its loads
 // happen before its first barrier and all of its stores after its last.
-template <class V, int SEG, int AXIS, bool FULL = false>
+template <class V, int SEG, int AXIS, bool FULL = false, bool RY = false>
 __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const double *Tn,
                                                    const double *Y, double *X, double *surf,
                                                    double dz, int bx, int by) {
@@ -560,10 +604,12 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
   typename V::Raw raw[SEG + 1];
   if constexpr (FULL) {  // segments tile the line exactly: only the halo cells clamp
     const int b0 = base + t0 * stride;
-    T[0] = Tn[base + max(t0 - 1, 0) * stride];
+    if constexpr (!RY) {
+      T[0] = Tn[base + max(t0 - 1, 0) * stride];
 #pragma unroll
-    for (int q = 1; q <= SEG; q++) T[q] = Tn[b0 + (q - 1) * stride];
-    T[SEG + 1] = Tn[base + min(t0 + SEG, L - 1) * stride];
+      for (int q = 1; q <= SEG; q++) T[q] = Tn[b0 + (q - 1) * stride];
+      T[SEG + 1] = Tn[base + min(t0 + SEG, L - 1) * stride];
+    }
 #pragma unroll
     for (int q = 0; q < SEG; q++) Yv[q] = Y[b0 + q * stride];
 #pragma unroll
@@ -593,9 +639,10 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
     const double gm = (FULL ? (q > 0 || p > 0) : t >= 1) ? en.g : 0.0;
     const double gp =
         (FULL ? (q < SEG - 1 || t0 + SEG < L) : t + 1 <= L - 1) ? (V::kAsym ? en.gp : ep.g) : 0.0;
-    const double Tc = T[q + 1];
     const double M = en.M, w = en.w;
-    const double rhs = M * Yv[q] - 0.5 * w * (gm * (T[q] - Tc) + gp * (T[q + 2] - Tc));
+    // RY: Y already holds M T* - 0.5 w (Ly T^n), written by the x sweep
+    const double rhs =
+        RY ? Yv[q] : M * Yv[q] - 0.5 * w * (gm * (T[q] - T[q + 1]) + gp * (T[q + 2] - T[q + 1]));
     const double am = -0.5 * w * gm, cp = -0.5 * w * gp;
     const double inv = 1.0 / (M - am - cp);
     if constexpr (FULL) {
@@ -631,7 +678,7 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
   if (surf && axis == 0 && p == 0 && L >= 2) surf[base] = surface_readout(x[0], x[1], dz);
 }
 
-template <class V, int SEG, int AXIS, bool FULL>
+template <class V, int SEG, int AXIS, bool FULL, bool RY = false>
 __global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, const double *Tn,
                                                        const double *Y, double *X, double *surf,
                                                        double dz) {
@@ -639,8 +686,8 @@ __global__ __launch_bounds__(512) void k_sweep_strided(Grid g, V view, int CW, c
   // rows: keep them on one XCD (one L2)
   const int nb = gridDim.x * gridDim.y;
   const int r = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
-  sweep_strided_body<V, SEG, AXIS, FULL>(g, view, CW, Tn, Y, X, surf, dz, r % gridDim.x,
-                                         r / gridDim.x);
+  sweep_strided_body<V, SEG, AXIS, FULL, RY>(g, view, CW, Tn, Y, X, surf, dz, r % gridDim.x,
+                                             r / gridDim.x);
 }
 
 // ------------------------------------------------------ K0: classify
@@ -703,10 +750,23 @@ static hipError_t launch_x_t(const Grid &g, const V &v, const double *Tn, double
   return hipGetLastError();
 }
 
+// FULL two-phase launch: lines fill the lanes exactly, every block holds
+// existing lines (whole tiles), and the field fits 32-bit byte offsets
+static bool x2_full(const Grid &g, int lanes_cells, int lpb, int ty) {
+  const int nlines = g.nz * g.ny;
+  const bool whole = ty > 0 ? g.nz % (lpb / ty) == 0 && g.ny % ty == 0 : nlines % lpb == 0;
+  return g.nx == lanes_cells && whole &&
+         (size_t)nlines * g.nx * sizeof(double) < (size_t(1) << 32);
+}
+
+bool x_ry_ok(XConfig c, const Grid &g) {
+  return c.lpb > 0 && c.cpl % 4 == 0 && x2_full(g, c.cpl * c.g, c.lpb, c.ty);
+}
+
 template <class V, int SEG, int G, int LPB>
 static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *Tn, double *Ts,
                               const uint8_t *vmap, const double *vtab, const double *sfield,
-                              hipStream_t s) {
+                              bool ry, hipStream_t s) {
   const int nlines = g.nz * g.ny;
   const int tz = ty > 0 ? LPB / ty : 0;
   const int blocks = ty > 0 ? ((g.nz + tz - 1) / tz) * ((g.ny + ty - 1) / ty)
@@ -715,10 +775,16 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *T
   const bool src = vmap || sfield;
   const size_t shm =
       sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + (src ? kMaxVtab : 0) + LPB * RS);
-  // FULL: lines fill the lanes exactly and every block holds existing lines
-  const bool whole = ty > 0 ? g.nz % tz == 0 && g.ny % ty == 0 : nlines % LPB == 0;
-  const bool full = g.nx == SEG * G && whole &&
-                    (size_t)nlines * g.nx * sizeof(double) < (size_t(1) << 32);
+  const bool full = x2_full(g, SEG * G, LPB, ty);
+  if constexpr (std::is_same<V, ClassView>::value && SEG % 4 == 0) {
+    if (ry) {  // the caller checked x_ry_ok and that the step has no sources
+      if (!full || src) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, true, true>), dim3(blocks),
+                         dim3(G * LPB), shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
+      return hipGetLastError();
+    }
+  }
+  if (ry) return hipErrorInvalidValue;
   if (src)
     hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, true, false>), dim3(blocks), dim3(G * LPB), shm,
                        s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
@@ -750,7 +816,8 @@ XConfig x_config_for(int nx, int nz) {
 template <class V>
 static hipError_t launch_x_v(XConfig cfg, const Grid &g, const V &v, const double *Tn,
                              double *Ts, const uint8_t *vmap, const double *vtab,
-                             const double *sfield, hipStream_t s) {
+                             const double *sfield, bool ry, hipStream_t s) {
+  if (ry && cfg.lpb == 0) return hipErrorInvalidValue;
 #define HS2_X(C, GG)                                                               \
   if (cfg.lpb == 0 && cfg.cpl == C && cfg.g == GG)                                 \
     return launch_x_t<V, C, GG>(g, v, Tn, Ts, vmap, vtab, sfield, s);
@@ -760,7 +827,7 @@ static hipError_t launch_x_v(XConfig cfg, const Grid &g, const V &v, const doubl
 #undef HS2_X
 #define HS2_X2(C, GG, LL)                                                          \
   if (cfg.lpb == LL && cfg.cpl == C && cfg.g == GG)                                \
-    return launch_x2_t<V, C, GG, LL>(g, v, cfg.ty, Tn, Ts, vmap, vtab, sfield, s);
+    return launch_x2_t<V, C, GG, LL>(g, v, cfg.ty, Tn, Ts, vmap, vtab, sfield, ry, s);
   HS2_X2(1, 16, 8) HS2_X2(2, 16, 8) HS2_X2(4, 16, 8) HS2_X2(8, 16, 8) HS2_X2(16, 16, 8)
   HS2_X2(4, 16, 16) HS2_X2(8, 16, 16) HS2_X2(16, 16, 4) HS2_X2(16, 16, 16)
   HS2_X2(8, 32, 4) HS2_X2(16, 32, 4) HS2_X2(16, 32, 2) HS2_X2(16, 64, 2) HS2_X2(8, 64, 2)
@@ -770,18 +837,18 @@ static hipError_t launch_x_v(XConfig cfg, const Grid &g, const V &v, const doubl
 
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const ClassView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
-                          const double *sfield, hipStream_t s) {
-  return launch_x_v(cfg, g, v, Tn, Ts, vmap, vtab, sfield, s);
+                          const double *sfield, hipStream_t s, bool ry) {
+  return launch_x_v(cfg, g, v, Tn, Ts, vmap, vtab, sfield, ry, s);
 }
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const DenseView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
-                          const double *sfield, hipStream_t s) {
-  return launch_x_v(cfg, g, v, Tn, Ts, vmap, vtab, sfield, s);
+                          const double *sfield, hipStream_t s, bool ry) {
+  return launch_x_v(cfg, g, v, Tn, Ts, vmap, vtab, sfield, ry, s);
 }
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const AsymView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
-                          const double *sfield, hipStream_t s) {
-  return launch_x_v(cfg, g, v, Tn, Ts, vmap, vtab, sfield, s);
+                          const double *sfield, hipStream_t s, bool ry) {
+  return launch_x_v(cfg, g, v, Tn, Ts, vmap, vtab, sfield, ry, s);
 }
 
 template <class V>
@@ -797,7 +864,8 @@ size_t strided_lds_bytes(const DenseView &v, int axis, int P, int CW) {
 
 template <class V, int SEG>
 static hipError_t launch_s_t(const Grid &g, const V &v, int axis, int CW, const double *Tn,
-                             const double *Y, double *X, double *surf, double dz, hipStream_t s) {
+                             const double *Y, double *X, double *surf, double dz, bool ry,
+                             hipStream_t s) {
   const int L = axis == 1 ? g.ny : g.nz;
   const int P = (L + SEG - 1) / SEG;
   const int other = axis == 1 ? g.nz : g.ny;
@@ -806,6 +874,19 @@ static hipError_t launch_s_t(const Grid &g, const V &v, int axis, int CW, const 
   // FULL: the segments tile the line and the column blocks tile nx
   const bool full = P * SEG == L && g.nx % CW == 0;
   const dim3 block(P * CW);
+  if (ry) {  // y from the x sweep's R_y (class maps only)
+    if constexpr (std::is_same<V, ClassView>::value) {
+      if (axis != 1) return hipErrorInvalidValue;
+      if (full)
+        hipLaunchKernelGGL((k_sweep_strided<V, SEG, 1, true, true>), grid, block, lds, s, g, v, CW,
+                           Tn, Y, X, surf, dz);
+      else
+        hipLaunchKernelGGL((k_sweep_strided<V, SEG, 1, false, true>), grid, block, lds, s, g, v,
+                           CW, Tn, Y, X, surf, dz);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+  }
   if (axis == 1) {
     if (full)
       hipLaunchKernelGGL((k_sweep_strided<V, SEG, 1, true>), grid, block, lds, s, g, v, CW, Tn, Y,
@@ -826,31 +907,32 @@ static hipError_t launch_s_t(const Grid &g, const V &v, int axis, int CW, const 
 
 template <class V>
 static hipError_t launch_s_v(SConfig cfg, const Grid &g, const V &v, int axis, const double *Tn,
-                             const double *Y, double *X, double *surf, double dz, hipStream_t s) {
+                             const double *Y, double *X, double *surf, double dz, bool ry,
+                             hipStream_t s) {
   switch (cfg.seg) {
-    case 2: return launch_s_t<V, 2>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, s);
-    case 4: return launch_s_t<V, 4>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, s);
-    case 8: return launch_s_t<V, 8>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, s);
-    case 16: return launch_s_t<V, 16>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, s);
-    case 32: return launch_s_t<V, 32>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, s);
+    case 2: return launch_s_t<V, 2>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, ry, s);
+    case 4: return launch_s_t<V, 4>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, ry, s);
+    case 8: return launch_s_t<V, 8>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, ry, s);
+    case 16: return launch_s_t<V, 16>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, ry, s);
+    case 32: return launch_s_t<V, 32>(g, v, axis, cfg.cw, Tn, Y, X, surf, dz, ry, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const ClassView &v, int axis,
                                 const double *Tn, const double *Y, double *X, double *surf,
-                                double dz, hipStream_t s) {
-  return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, s);
+                                double dz, hipStream_t s, bool ry) {
+  return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, ry, s);
 }
 hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const DenseView &v, int axis,
                                 const double *Tn, const double *Y, double *X, double *surf,
-                                double dz, hipStream_t s) {
-  return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, s);
+                                double dz, hipStream_t s, bool ry) {
+  return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, ry, s);
 }
 hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const AsymView &v, int axis,
                                 const double *Tn, const double *Y, double *X, double *surf,
-                                double dz, hipStream_t s) {
-  return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, s);
+                                double dz, hipStream_t s, bool ry) {
+  return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, ry, s);
 }
 
 hipError_t launch_classify(const Grid &g, const uint8_t *mat, const uint8_t *bz, const uint8_t *by,

@@ -15,29 +15,33 @@ struct SConfig {  // strided sweep: SEG cells per thread, CW lines per block
 };
 
 XConfig x_config_for(int nx, int nz);  // {0,0}: unsupported (nx > 1024)
+// ry: the x sweep writes y's right-hand side M T* - (Ly/2) T^n instead of T*
+// and the y sweep reads it (class maps, steps without sources, x shapes for
+// which x_ry_ok holds); the z sweep is unchanged
+bool x_ry_ok(XConfig c, const Grid &g);
 
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const ClassView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
-                          const double *sfield, hipStream_t s);
+                          const double *sfield, hipStream_t s, bool ry = false);
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const DenseView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
-                          const double *sfield, hipStream_t s);
+                          const double *sfield, hipStream_t s, bool ry = false);
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const AsymView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
-                          const double *sfield, hipStream_t s);
+                          const double *sfield, hipStream_t s, bool ry = false);
 
 size_t strided_lds_bytes(const ClassView &v, int axis, int P, int CW);
 size_t strided_lds_bytes(const DenseView &v, int axis, int P, int CW);
 
 hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const ClassView &v, int axis,
                                 const double *Tn, const double *Y, double *X, double *surf,
-                                double dz, hipStream_t s);
+                                double dz, hipStream_t s, bool ry = false);
 hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const DenseView &v, int axis,
                                 const double *Tn, const double *Y, double *X, double *surf,
-                                double dz, hipStream_t s);
+                                double dz, hipStream_t s, bool ry = false);
 hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const AsymView &v, int axis,
                                 const double *Tn, const double *Y, double *X, double *surf,
-                                double dz, hipStream_t s);
+                                double dz, hipStream_t s, bool ry = false);
 
 hipError_t launch_classify(const Grid &g, const uint8_t *mat, const uint8_t *bz, const uint8_t *by,
                            const uint8_t *bx, const uint8_t *lut, const uint8_t *edge, int nmat,

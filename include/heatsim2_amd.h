@@ -115,8 +115,11 @@ int hs2_plan_expand(hs2_plan *plan, double *d_M, double *d_w, double *d_gz, doub
 int hs2_plan_set_config(hs2_plan *plan, int x_cpl, int x_lanes, int x_lines, int x_lines_y,
                         int y_seg, int y_lines, int z_seg, int z_lines);
 
-/* info[0..11] = mode (0 none, 1 class maps, 2 dense), classes per axis z/y/x,
- * x_cpl, x_lanes, y_seg, y_lines, z_seg, z_lines, x_lines, x_lines_y. */
+/* info[0..12] = mode (0 none, 1 class maps, 2 dense, 3 dense per side),
+ * classes per axis z/y/x, x_cpl, x_lanes, y_seg, y_lines, z_seg, z_lines,
+ * x_lines, x_lines_y, and 1 when steps without sources hand the y sweep its
+ * right-hand side M T* - (Ly/2) T^n from the x sweep (16 B per cell less field
+ * traffic per step than writing T* and re-reading T^n), else 0. */
 int hs2_plan_info(const hs2_plan *plan, int *info);
 
 /* One full Douglas step T^n -> T^{n+1} (x, y, z substeps).
