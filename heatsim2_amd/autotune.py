@@ -125,15 +125,20 @@ def tune(ADI_params, T=None, steps=4, use_cache=True, verbose=False):
             if not _apply(L, plan, trial, strict=False):
                 continue
             _time(L, plan, scratch, 1)  # warm the code object
-            ms = _time(L, plan, scratch, steps)[idx]
+            t = _time(L, plan, scratch, steps)
+            # the x shape also decides whether y reads x's right-hand side
+            # (hs2_plan_info "y_from_x"), so x is scored with the y sweep
+            ms = t[0] + t[1] if axis_name == "x" else t[idx]
             results.append((ms, cand))
             if verbose:
-                print("autotune %s %s: %.4f ms" % (axis_name, cand, ms), flush=True)
+                print("autotune %s %s: %.4f ms%s" % (axis_name, cand, ms,
+                                                     " (x + y)" if axis_name == "x" else ""),
+                      flush=True)
         if results:
             ms, cand = min(results)
             cur[axis_name] = cand
-            best_ms[idx] = ms
         _apply(L, plan, cur)
+        best_ms[idx] = _time(L, plan, scratch, steps)[idx] if results else None
     cfg = {"x": list(cur["x"]), "y": list(cur["y"]), "z": list(cur["z"]), "ms": best_ms}
     _memo[key] = cfg
     if use_cache:

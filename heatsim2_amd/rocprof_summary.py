@@ -24,6 +24,9 @@ import re
 import sys
 
 BYTES_PER_CELL = {"x": 16.0, "y": 24.0, "z": 24.0}
+# y from the x sweep's right-hand side (the strided kernel's fifth template
+# argument): reads R_y and writes T**, no T^n
+Y_FROM_X_BYTES = 16.0
 
 
 def _one(directory, suffix):
@@ -45,6 +48,7 @@ def short_name(name):
 
 
 _STRIDED = re.compile(r"k_sweep_strided<[^,<>]+,\s*\d+,\s*(\d+)")
+_STRIDED_RY = re.compile(r"k_sweep_strided<[^,<>]+,\s*\d+,\s*1,\s*(?:true|false),\s*true>")
 
 
 def sweep_of(name):
@@ -58,17 +62,29 @@ def sweep_of(name):
     return None
 
 
-def sweep_of_dispatches(dispatches):
-    """dispatches: [(dispatch_id, kernel_name, value)] -> {"x"|"y"|"z": [values]}.
+def algo_bytes_per_cell(sweep, name):
+    """Algorithmic FP64 field bytes per cell of a sweep kernel instantiation."""
+    if sweep == "y" and _STRIDED_RY.search(name):
+        return Y_FROM_X_BYTES
+    return BYTES_PER_CELL[sweep]
 
-    Per sweep only the most frequently launched instantiation counts (the
-    source-free step kernel; the flash step's source variant is excluded)."""
+
+def sweep_kernels(dispatches):
+    """dispatches: [(dispatch_id, kernel_name, value)] -> {"x"|"y"|"z":
+    (kernel_name, [values])} of the most frequently launched instantiation
+    per sweep (the source-free step kernel; the flash step's source variant
+    is excluded)."""
     per = collections.defaultdict(lambda: collections.defaultdict(list))
     for _, name, value in sorted(dispatches):
         s = sweep_of(name)
         if s:
             per[s][name].append(value)
-    return {s: max(by.values(), key=len) for s, by in per.items()}
+    return {s: max(by.items(), key=lambda kv: len(kv[1])) for s, by in per.items()}
+
+
+def sweep_of_dispatches(dispatches):
+    """dispatches -> {"x"|"y"|"z": [values]} (see sweep_kernels)."""
+    return {s: v for s, (_, v) in sweep_kernels(dispatches).items()}
 
 
 def trace_summary(directory):
@@ -125,7 +141,8 @@ def traffic_summary(a):
     lines.append("calibration: %s over %d B: FETCH_SIZE %.0f KiB (n=%d), WRITE_SIZE %.0f KiB (n=%d)"
                  % (cname, a.copy_bytes, cf, nf, cw, nw))
     lines.append("scale factors: fetch x%.3f, write x%.3f" % (ff, fw))
-    fetch = sweep_of_dispatches(_counter(a.fetch, "FETCH_SIZE"))
+    fk = sweep_kernels(_counter(a.fetch, "FETCH_SIZE"))
+    fetch = {s: v for s, (_, v) in fk.items()}
     write = sweep_of_dispatches(_counter(a.write, "WRITE_SIZE"))
     lines.append("")
     lines.append("%-8s %12s %12s %14s %14s %14s %10s" % (
@@ -137,7 +154,7 @@ def traffic_summary(a):
         f = sum(fetch[s]) / len(fetch[s])
         w = sum(write[s]) / len(write[s])
         rb, wb = f * 1024.0 * ff, w * 1024.0 * fw
-        algo = BYTES_PER_CELL[s] * a.cells
+        algo = algo_bytes_per_cell(s, fk[s][0]) * a.cells
         out["sweep_" + s] = rb + wb
         lines.append("%-8s %12.0f %12.0f %14.1f %14.1f %14.1f %10.1f" % (
             "sweep_" + s, f, w, rb / 1e6, wb / 1e6, (rb + wb) / 1e6, algo / 1e6))

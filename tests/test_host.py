@@ -212,6 +212,12 @@ def test_rocprof_summary_attribution_and_committed_traffic():
     assert rocprof_summary.sweep_of("void hs2::k_sweep_strided<hs2::ClassView, 8, 0, false>(x)") == "z"
     assert rocprof_summary.sweep_of("void hs2::k_sweep_x2<hs2::ClassView, 16, 16, 4, false>") == "x"
     assert rocprof_summary.sweep_of("hs2::k_classify") is None
+    ry = "void hs2::k_sweep_strided<hs2::ClassView, 16, 1, true, true>(x)"
+    assert rocprof_summary.sweep_of(ry) == "y"
+    assert rocprof_summary.algo_bytes_per_cell("y", ry) == 16.0
+    assert rocprof_summary.algo_bytes_per_cell("y", ry.replace("true>", "false>")) == 24.0
+    assert rocprof_summary.algo_bytes_per_cell("x", "void hs2::k_sweep_x2<hs2::ClassView, 16, "
+                                                    "16, 16, false, true, true>") == 16.0
     tr = rocprof_summary.read_traffic(os.path.join(ROOT, "profiles",
                                                    "hbm_traffic_two_layer_256.txt"))
     assert tr["cells"] == 256 ** 3
