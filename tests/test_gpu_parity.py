@@ -222,6 +222,31 @@ def test_curved_random_problems_match_oracle(hs, shape):
     assert torch.equal(T, T2)
 
 
+def test_y_from_x_path_matches_oracle(hs):
+    """Full-line x shapes on a class-map plan hand the y sweep its right-hand
+    side M T* - (Ly/2) T^n (hs2_plan_info "y_from_x"); steps with sources
+    and other shapes keep T*.  Both agree with the oracle, step by step."""
+    from heatsim2_amd import _lib, autotune
+    case = _random_case((8, 16, 128), seed=5)
+    case["nsteps"] = 6  # the stepped source ends after step 3
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    ref = oracle.run_ref(case)
+    rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
+    info = _lib.plan_info(P.plan)
+    assert info["y_from_x"] == 1
+    base = {"y": (info["y_seg"], info["y_lines"]), "z": (info["z_seg"], info["z_lines"])}
+    for x, expect in (((8, 16, 8, 2), 1), ((8, 16, 8, 0), 1), ((4, 32, 0, 0), 0),
+                      ((16, 16, 8, 0), 0)):
+        autotune._apply(_lib.load(), P.plan, dict(base, x=x))
+        assert _lib.plan_info(P.plan)["y_from_x"] == expect, x
+        T = torch.from_numpy(case["T0"]).cuda()
+        for n in range(case["nsteps"]):
+            T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"],
+                                 args["volumetric"])
+            assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, (x, n)
+
+
 def test_run_many_equals_stepwise(hs):
     case = load_golden("tiny_mixed")
     args = to_package_args(hs, case)
