@@ -4,8 +4,9 @@ two cells' 3x3 conductivities in (z, y, x) order.
 Same face law as the upstream boundary_conducting_anisotropic.py:7-67: the
 normal derivative is a two-point difference, the tangential derivatives are
 four-point averages over both cells.  With a diagonal K only the normal term
-survives; off-diagonal K needs the 15-substep unaligned mode, which this
-package does not provide (setup() rejects it).
+survives.  Off-diagonal K needs the 15-substep unaligned mode, whose
+upstream setup fails an assert before any step exists; setup() here fails
+the same way (an AssertionError subclass).
 """
 from heatsim2_amd.expression import group
 
@@ -32,6 +33,10 @@ def qx(kmat55m, kmat55p, dz, dy, dx, T55m, T55p, T45m, T45p, T65m, T65p, T54m, T
        T56m, T56p, T46m, T46p, T64m, T64p):
     K = (kmat55m + kmat55p) * 0.5
     dTdz = group((T65p - T45p) * (0.25 / dz)) + group((T65m - T45m) * (0.25 / dz))
-    dTdy = group((T56p - T54p) * (0.25 / dy)) + group((T56m - T54m) * (0.25 / dy))
+    # the minus cell's y difference is T55m - T54m upstream, not T56m - T54m
+    # (boundary_conducting_anisotropic.py:60); kept, since it is the face law
+    # a plugin caller sees, and K[2, 1] = 0 makes it drop out of every case
+    # setup() accepts
+    dTdy = group((T56p - T54p) * (0.25 / dy)) + group((T55m - T54m) * (0.25 / dy))
     dTdx = group((T55p - T55m) * (1.0 / dx))
     return -K[2, 0] * dTdz - K[2, 1] * dTdy - K[2, 2] * dTdx

@@ -46,6 +46,26 @@ _AXIS_NAME = ("z", "y", "x")
 MAX_TABLE_DOUBLES = 4096
 
 
+class UnalignedAnisotropicError(NotImplementedError, AssertionError):
+    """setup(..., unaligned_anisotropic=True).  The upstream fails this mode
+    with an AssertionError before any step exists: adi_expressions returns 15
+    substep expressions (alternatingdirection_c_pyx.pyx:511-564) while
+    adi_setup always builds 3 steps (:418-445), so
+    add_equation_to_adi_matrices fails `len(ADI_steps) ==
+    len(spatial_expressions)` at the first cell (:452).  Substeps 3-14 would
+    also have no line to solve on: their unknowns (e.g. T656p3, T654p3) are
+    shifted on two axes, which add_equation asserts against
+    (alternatingdirection_c.c:146), and sit in another plane than the
+    substep's own T555p3.  Raised as both AssertionError, as upstream, and
+    NotImplementedError."""
+
+
+class OffDiagonalConductivityError(ValueError, AssertionError):
+    """A face flux that depends on a tangential difference (off-diagonal
+    conductivity) without the unaligned mode.  The upstream fails
+    `assert no_groups(...)` here (alternatingdirection_c_pyx.pyx:570)."""
+
+
 class adi_params(object):
     """Plan for one grid (alternatingdirection_c_pyx.pyx:75-85 fields plus
     the device state)."""
@@ -150,9 +170,10 @@ class _FaceLaws(object):
             elif name == tm:
                 beta += coef
             else:
-                raise ValueError("%s: the flux depends on %s (off-diagonal conductivity); that "
-                                 "needs unaligned_anisotropic mode, which is not supported"
-                                 % (where, name))
+                raise OffDiagonalConductivityError(
+                    "%s: the flux depends on %s (off-diagonal conductivity); that needs the "
+                    "unaligned_anisotropic mode, which the upstream cannot run either"
+                    % (where, name))
         if abs(alpha + beta) > 1e-12 * max(abs(alpha), abs(beta)):
             raise ValueError("%s: flux %g*T_plus + %g*T_minus is not a two-point difference"
                              % (where, alpha, beta))
@@ -364,15 +385,19 @@ def setup(z0, y0, x0,
     z0, y0, x0 are accepted and unused, as upstream.  With both top-surface
     curvature arrays (ny, nx) given, the curvature mode of
     crank_nicolson.pyx:388-457 (per-cell areas and volumes; the volumes also
-    scale point sources).  The unaligned-anisotropic mode is not provided.
+    scale point sources).  unaligned_anisotropic=True raises
+    UnalignedAnisotropicError, an AssertionError as upstream, whose setup
+    cannot complete in that mode.
     """
     curved = top_surface_y_curvatures is not None or top_surface_x_curvatures is not None
     if curved and (top_surface_y_curvatures is None or top_surface_x_curvatures is None):
         raise TypeError("curvature mode needs both top_surface_y_curvatures and "
                         "top_surface_x_curvatures (the upstream indexes both)")
     if unaligned_anisotropic:
-        raise NotImplementedError("unaligned_anisotropic (off-diagonal K) mode is not provided "
-                                  "by heatsim2_amd")
+        raise UnalignedAnisotropicError(
+            "unaligned_anisotropic mode: the upstream builds 3 ADI steps for 15 substep "
+            "expressions and fails its assert at the first cell; there is no result to "
+            "reproduce")
     for v in (nz, ny, nx):
         if not isinstance(v, (numbers.Integral, np.integer)) or v < 1:
             raise ValueError("nz, ny, nx must be positive integers")

@@ -110,10 +110,36 @@ def test_setup_rejects_wrong_conductivity_type():
         hs.setup(*a)
 
 
-def test_setup_modes_not_provided():
-    a = _setup_args()
+@pytest.mark.parametrize("name", ["tiny_mixed", "tiny_aniso"])
+def test_unaligned_anisotropic_fails_as_upstream(name):
+    """The upstream's setup cannot complete in this mode: adi_setup builds 3
+    steps, adi_expressions returns 15, and add_equation_to_adi_matrices
+    asserts they agree (alternatingdirection_c_pyx.pyx:418-452, 511-564).
+    The same call fails here with an AssertionError, before any device work."""
+    a = _setup_args(name)
+    with pytest.raises(AssertionError, match="15 substep"):
+        hs.setup(*a, unaligned_anisotropic=True)
     with pytest.raises(NotImplementedError):
         hs.setup(*a, unaligned_anisotropic=True)
+
+
+@pytest.mark.parametrize("entry", [(0, 1), (0, 2), (1, 2), (2, 1)])
+def test_off_diagonal_conductivity_fails_as_upstream(entry):
+    """Off-diagonal K without the unaligned mode leaves tangential groups in
+    the last substep, and the upstream fails `assert no_groups`
+    (alternatingdirection_c_pyx.pyx:570).  Each off-diagonal entry alone is
+    caught, (2, 1) included, whose qx term is T56p - T54p and T55m - T54m
+    upstream (boundary_conducting_anisotropic.py:60)."""
+    a = _setup_args("tiny_aniso")
+    mats = list(a[10])
+    k = np.array(mats[0][1], dtype=np.float64).copy()
+    k[entry] = 0.1 * k[entry[0], entry[0]]
+    mats[0] = (mats[0][0], k) + tuple(mats[0][2:])
+    a[10] = tuple(mats)
+    with pytest.raises(AssertionError, match="off-diagonal"):
+        hs.setup(*a)
+    with pytest.raises(ValueError):
+        hs.setup(*a)
 
 
 def test_curvature_setup_arguments():
