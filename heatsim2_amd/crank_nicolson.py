@@ -98,7 +98,10 @@ class adi_params(object):
 class adi_step(object):
     """Descriptor of one Douglas substep (alternatingdirection_c_pyx.pyx:
     120-167): the axis it solves and the upstream's permutation.  The
-    operators themselves live on the device."""
+    operators live on the device as per-cell coefficients; Amat, Bmat, Cmats,
+    Dvec, Lmat and Umat give them in the upstream's assembled form on first
+    access (alternatingdirection_c_pyx.pyx:212-282), for inspection and
+    comparison.  The step never forms them."""
 
     def __init__(self, ADI_params, permuteorder, stepnum):
         self.ADI_params = ADI_params
@@ -108,6 +111,180 @@ class adi_step(object):
         inv = sorted((p, i) for i, p in enumerate(permuteorder))
         self.invpermuteorder = tuple(i for _, i in inv)
         self.axis = ("x", "y", "z")[stepnum]
+        self._ops = None
+        self._lu = None
+
+    def _operators(self):
+        if self._ops is None:
+            M, D, gm, gp = _equation_coefficients(self.ADI_params)
+            self._ops = reference_operators(self.ADI_params.shape, self.stepnum, M, D, gm, gp)
+        return self._ops
+
+    @property
+    def Amat(self):
+        """(n, 3): row r of this substep's tridiagonal matrix, entries for the
+        line neighbour before, the cell, the neighbour after."""
+        return self._operators()[0]
+
+    @property
+    def Bmat(self):
+        """CSR (n, n) on T^n, in this substep's row and column order."""
+        return self._operators()[1]
+
+    @property
+    def Cmats(self):
+        """CSR (n, n) on each earlier substep's result."""
+        return self._operators()[2]
+
+    @property
+    def Dvec(self):
+        """(n,): weight of the volumetric source in each row."""
+        return self._operators()[3]
+
+    @property
+    def Lmat(self):
+        """Thomas factor L of Amat as tridiag.tridiaglu lays it out."""
+        if self._lu is None:
+            self._lu = tridiaglu(self.Amat, self.permutedshape[2])
+        return self._lu[0]
+
+    @property
+    def Umat(self):
+        """Thomas factor U of Amat as tridiag.tridiaglu lays it out."""
+        if self._lu is None:
+            self._lu = tridiaglu(self.Amat, self.permutedshape[2])
+        return self._lu[1]
+
+
+# array axis (z, y, x) -> the substep whose Crank-Nicolson split makes it
+# implicit (adi_expressions: x first, then y, then z)
+_SUBSTEP_OF_AXIS = (2, 1, 0)
+_PERMUTATIONS = ((0, 1, 2), (2, 0, 1), (1, 2, 0))
+
+
+def reference_operators(shape, stepnum, M, D, gm, gp):
+    """A (n x 3), B (CSR), [C] (CSR), D of Douglas substep `stepnum` in the
+    upstream's assembled form, from per-cell equation coefficients.
+
+    M is each row's time-term weight (rho c V/dt, 1 for a FIXED cell), D its
+    source weight, gm[a] / gp[a] the weight of its minus / plus face along
+    array axis a (z, y, x), zero on outer faces and in FIXED rows.  In the
+    upstream, a face term g (T_nb - T_c) of an axis already split by this or
+    an earlier substep s' reads g/2 (T_nb,p(s') + T_nb,m - T_c,p(s') - T_c,m)
+    (expression.py:753-827); later axes keep g (T_nb - T_c) at the old level
+    (expression.py:712-716).  Unknowns of this substep go to A with the sign
+    flipped, old-level terms to B, terms in an earlier substep's unknowns to
+    C[s'], and the source weight to D (alternatingdirection_c.c:102-199).
+    Rows and columns follow the substep's permutation
+    (alternatingdirection_c_pyx.pyx:418-445, alternatingdirection_c.c:114).
+    The upstream's ADI_steps[s] hold the same operators."""
+    import scipy.sparse as sp
+    shape = tuple(int(v) for v in shape)
+    n = int(np.prod(shape))
+    perm = _PERMUTATIONS[stepnum]
+    ps = [shape[p] for p in perm]
+    row = np.arange(n).reshape(ps).transpose(np.argsort(perm))  # equation of cell (k, j, i)
+    M = np.broadcast_to(np.asarray(M, dtype=np.float64), shape)
+    D = np.broadcast_to(np.asarray(D, dtype=np.float64), shape)
+    A = np.zeros((n, 3))
+    Dvec = np.zeros(n)
+    r_all = row.ravel()
+    A[r_all, 1] = M.ravel()
+    Dvec[r_all] = D.ravel()
+    B = [(r_all, r_all, M.ravel())]
+    C = [[] for _ in range(stepnum)]
+    for a in range(3):
+        sub = _SUBSTEP_OF_AXIS[a]
+        for side, g in ((-1, gm[a]), (1, gp[a])):
+            g = np.broadcast_to(np.asarray(g, dtype=np.float64), shape)
+            own = [slice(None)] * 3
+            nb = [slice(None)] * 3
+            own[a] = slice(1, None) if side < 0 else slice(0, -1)
+            nb[a] = slice(0, -1) if side < 0 else slice(1, None)
+            gf = g[tuple(own)]
+            keep = gf != 0.0
+            r = row[tuple(own)][keep]
+            cn = row[tuple(nb)][keep]
+            gf = gf[keep]
+            if sub > stepnum:  # not split yet: old level in full
+                B += [(r, r, -gf), (r, cn, gf)]
+                continue
+            h = 0.5 * gf
+            B += [(r, r, -h), (r, cn, h)]
+            if sub == stepnum:
+                A[r, 1] += h
+                A[r, 1 + side] = -h
+            else:
+                C[sub] += [(r, r, -h), (r, cn, h)]
+
+    def csr(parts):
+        if not parts:
+            return sp.csr_matrix((n, n))
+        rows, cols, vals = (np.concatenate(x) for x in zip(*parts))
+        return sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+
+    return A, csr(B), [csr(c) for c in C], Dvec
+
+
+def tridiaglu(Amat, line):
+    """(Lmat, Umat) of tridiag.tridiaglu (tridiag.pyx:9-43): Lmat[:, 0] the
+    sub-diagonal, Lmat[:, 1] the pivots d'_r = b_r - a_r c'_{r-1}, Umat[:, 1]
+    = 1 and Umat[:, 2] = c'_r = c_r / d'_r.  Lines of `line` rows are
+    independent (a = 0 at a line's first row makes the upstream's single
+    recurrence restart there exactly), so the recurrence runs across all
+    lines at once."""
+    A = np.asarray(Amat, dtype=np.float64)
+    n = A.shape[0]
+    a = A[:, 0].reshape(-1, line)
+    b = A[:, 1].reshape(-1, line)
+    c = A[:, 2].reshape(-1, line)
+    d = np.empty_like(b)
+    cp = np.empty_like(b)
+    d[:, 0] = b[:, 0]
+    cp[:, 0] = c[:, 0] / d[:, 0]
+    for q in range(1, line):
+        d[:, q] = b[:, q] - cp[:, q - 1] * a[:, q]
+        cp[:, q] = c[:, q] / d[:, q]
+    Lmat = np.zeros((n, 3))
+    Lmat[:, 0] = A[:, 0]
+    Lmat[0, 0] = 0.0
+    Lmat[:, 1] = d.ravel()
+    Umat = np.zeros((n, 3))
+    Umat[:, 1] = 1.0
+    Umat[:, 2] = cp.ravel()
+    return Lmat, Umat
+
+
+def _equation_coefficients(params):
+    """Per-cell (M, D, gm[3], gp[3]) of a plan at equation level, read back
+    from the device: what each cell's equation holds, multiplied through by
+    its volume in curvature mode as the upstream does (crank_nicolson.pyx:
+    436-453)."""
+    shape = params.shape
+    if getattr(params, "curvatures", None) is not None:
+        c = [t.cpu().numpy() for t in params.d_coef]
+        comp = params.d_w.cpu().numpy() != 0
+        vol = np.broadcast_to(params.volume_array, shape)
+        M = np.where(comp, c[0] * vol, 1.0)
+        D = np.where(comp, vol, 0.0)
+        gm = [np.where(comp, c[1 + a] * vol, 0.0) for a in range(3)]
+        gp = [np.where(comp, c[4 + a] * vol, 0.0) for a in range(3)]
+        return M, D, gm, gp
+    f = [torch.zeros(shape, dtype=torch.float64, device=_lib.device()) for _ in range(5)]
+    _lib.check("hs2_plan_expand", _lib.load().hs2_plan_expand(
+        params.plan, *[_lib.ptr(t) for t in f], _lib.stream_handle()))
+    M, w, gz, gy, gx = [t.cpu().numpy() for t in f]
+    gm, gp = [], []
+    for a, g in enumerate((gz, gy, gx)):
+        up = np.zeros_like(g)
+        src = [slice(None)] * 3
+        dst = [slice(None)] * 3
+        src[a] = slice(1, None)
+        dst[a] = slice(0, -1)
+        up[tuple(dst)] = g[tuple(src)]
+        gm.append(g * w)
+        gp.append(up * w)
+    return M, w.astype(np.float64), gm, gp
 
 
 def _material_table(materials, dt):

@@ -247,6 +247,33 @@ def test_y_from_x_path_matches_oracle(hs):
             assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, (x, n)
 
 
+@pytest.mark.parametrize("name", ["tiny_mixed", "tiny_aniso", "tiny_curved"])
+def test_step_operators_match_upstream(hs, name):
+    """ADI_steps[s].Amat/Bmat/Cmats/Dvec of a device plan (per-cell
+    coefficients read back from the device) equal the upstream's assembled
+    matrices, and Lmat/Umat multiply back to Amat.  1e-14: curvature mode
+    keeps its coefficients per unit volume and multiplies the volume back."""
+    import scipy.sparse as sp
+    case = load_golden(name)
+    z = case["fixture"]
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    n = int(np.prod(case["shape"]))
+    for s, st in enumerate(S):
+        for mine, key in ((st.Amat, "A%d" % s), (st.Dvec, "D%d" % s)):
+            assert _rel(mine, z[key], np.abs(z[key]).max()) < 1e-14, key
+        assert len(st.Cmats) == s
+        for which, mine in enumerate([st.Bmat] + list(st.Cmats)):
+            key = "B%d" % s if which == 0 else "C%d_%d" % (s, which - 1)
+            ref = sp.csr_matrix((z[key + "_data"], z[key + "_indices"], z[key + "_indptr"]),
+                                shape=(n, n)).toarray()
+            assert _rel(mine.toarray(), ref, np.abs(ref).max()) < 1e-14, key
+        L, U = st.Lmat, st.Umat
+        prev = np.concatenate([[0.0], U[:-1, 2]])
+        LU = np.stack([L[:, 0], L[:, 0] * prev + L[:, 1], L[:, 1] * U[:, 2]], axis=1)
+        assert _rel(LU, st.Amat, np.abs(st.Amat).max()) < 1e-14, s
+
+
 def test_run_many_equals_stepwise(hs):
     case = load_golden("tiny_mixed")
     args = to_package_args(hs, case)

@@ -142,6 +142,46 @@ def test_off_diagonal_conductivity_fails_as_upstream(entry):
         hs.setup(*a)
 
 
+def _rel_dense(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b))) / max(float(np.abs(b).max()), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["tiny_mixed", "tiny_aniso", "tiny_curved"])
+def test_reference_operators_match_upstream(name):
+    """reference_operators (behind ADI_steps[s].Amat/Bmat/Cmats/Dvec), fed the
+    oracle's per-cell equation coefficients, gives the upstream's assembled
+    matrices of every substep (fixtures written by the real reference), and
+    tridiaglu's factors multiply back to Amat."""
+    import scipy.sparse as sp
+    case = load_golden(name)
+    z = case["fixture"]
+    shape = case["shape"]
+    coef = oracle.assemble_case(case).reshape(*shape, 8)
+    M, D = coef[..., 0], coef[..., 1]
+    gm = [coef[..., 2], coef[..., 4], coef[..., 6]]
+    gp = [coef[..., 3], coef[..., 5], coef[..., 7]]
+    n = M.size
+    for s in range(3):
+        A, B, C, Dv = crank_nicolson.reference_operators(shape, s, M, D, gm, gp)
+        assert _rel_dense(A, z["A%d" % s]) < 1e-15, s
+        assert _rel_dense(Dv, z["D%d" % s]) < 1e-15, s
+        assert len(C) == s
+        for which, mine in enumerate([B] + C):
+            key = "B%d" % s if which == 0 else "C%d_%d" % (s, which - 1)
+            ref = sp.csr_matrix((z[key + "_data"], z[key + "_indices"], z[key + "_indptr"]),
+                                shape=(n, n))
+            assert _rel_dense(mine.toarray(), ref.toarray()) < 1e-15, key
+        line = shape[(2, 1, 0)[s]]
+        L, U = crank_nicolson.tridiaglu(A, line)
+        assert (U[:, 1] == 1.0).all() and (U[:, 0] == 0.0).all() and (L[:, 2] == 0.0).all()
+        # (L U)[r, r-1] = L[r, 0]; (L U)[r, r] = L[r, 0] U[r-1, 2] + L[r, 1];
+        # (L U)[r, r+1] = L[r, 1] U[r, 2]
+        prev = np.concatenate([[0.0], U[:-1, 2]])
+        LU = np.stack([L[:, 0], L[:, 0] * prev + L[:, 1], L[:, 1] * U[:, 2]], axis=1)
+        assert _rel_dense(LU, A) < 1e-14, s
+
+
 def test_curvature_setup_arguments():
     """Both curvature arrays or none (the upstream indexes both,
     crank_nicolson.pyx:399); (ny, nx) shapes; edges stay positive."""
