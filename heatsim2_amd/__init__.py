@@ -21,6 +21,8 @@ import heatsim2_amd.surface_temperature as surface_temperature
 import heatsim2_amd.hs2_indexing as hs2_indexing
 import heatsim2_amd.crank_nicolson as crank_nicolson
 import heatsim2_amd.alternatingdirection as alternatingdirection
+import heatsim2_amd.alternatingdirection_c_pyx as alternatingdirection_c_pyx
+import heatsim2_amd.tridiag as tridiag
 import heatsim2_amd.autotune as autotune
 import heatsim2_amd.adjoint as adjoint
 import heatsim2_amd.batch as batch

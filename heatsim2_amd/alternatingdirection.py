@@ -15,6 +15,10 @@ import torch
 from heatsim2_amd import _lib
 from heatsim2_amd.constants import (IMPULSE_POINT_SOURCE_JOULES, IMPULSE_SOURCE,
                                     SPATIALLY_Z_DECAYING_TEMPORAL_IMPULSE, STEPPED_SOURCE)
+# the upstream module's plan and step classes (alternatingdirection_c_pyx.pyx:
+# 75-85, 120-284), under its names
+from heatsim2_amd.crank_nicolson import adi_params  # noqa: F401
+from heatsim2_amd.crank_nicolson import adi_step as pyadi_step  # noqa: F401
 
 
 def source_tables(ADI_params, t, dt, volumetric, volumetric_elements):

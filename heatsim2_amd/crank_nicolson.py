@@ -28,7 +28,7 @@ import numbers
 import numpy as np
 import torch
 
-from heatsim2_amd import _lib, expression
+from heatsim2_amd import _lib, expression, tridiag
 from heatsim2_amd.constants import TEMPERATURE_COMPUTE, TEMPERATURE_FIXED
 
 # placeholder names of each face law's temperature arguments; the first two
@@ -143,16 +143,16 @@ class adi_step(object):
 
     @property
     def Lmat(self):
-        """Thomas factor L of Amat as tridiag.tridiaglu lays it out."""
+        """Thomas factor L of Amat (tridiag.tridiaglu's layout)."""
         if self._lu is None:
-            self._lu = tridiaglu(self.Amat, self.permutedshape[2])
+            self._lu = tridiag.tridiaglu(self.Amat)
         return self._lu[0]
 
     @property
     def Umat(self):
-        """Thomas factor U of Amat as tridiag.tridiaglu lays it out."""
+        """Thomas factor U of Amat (tridiag.tridiaglu's layout)."""
         if self._lu is None:
-            self._lu = tridiaglu(self.Amat, self.permutedshape[2])
+            self._lu = tridiag.tridiaglu(self.Amat)
         return self._lu[1]
 
 
@@ -224,35 +224,6 @@ def reference_operators(shape, stepnum, M, D, gm, gp):
         return sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
 
     return A, csr(B), [csr(c) for c in C], Dvec
-
-
-def tridiaglu(Amat, line):
-    """(Lmat, Umat) of tridiag.tridiaglu (tridiag.pyx:9-43): Lmat[:, 0] the
-    sub-diagonal, Lmat[:, 1] the pivots d'_r = b_r - a_r c'_{r-1}, Umat[:, 1]
-    = 1 and Umat[:, 2] = c'_r = c_r / d'_r.  Lines of `line` rows are
-    independent (a = 0 at a line's first row makes the upstream's single
-    recurrence restart there exactly), so the recurrence runs across all
-    lines at once."""
-    A = np.asarray(Amat, dtype=np.float64)
-    n = A.shape[0]
-    a = A[:, 0].reshape(-1, line)
-    b = A[:, 1].reshape(-1, line)
-    c = A[:, 2].reshape(-1, line)
-    d = np.empty_like(b)
-    cp = np.empty_like(b)
-    d[:, 0] = b[:, 0]
-    cp[:, 0] = c[:, 0] / d[:, 0]
-    for q in range(1, line):
-        d[:, q] = b[:, q] - cp[:, q - 1] * a[:, q]
-        cp[:, q] = c[:, q] / d[:, q]
-    Lmat = np.zeros((n, 3))
-    Lmat[:, 0] = A[:, 0]
-    Lmat[0, 0] = 0.0
-    Lmat[:, 1] = d.ravel()
-    Umat = np.zeros((n, 3))
-    Umat[:, 1] = 1.0
-    Umat[:, 2] = cp.ravel()
-    return Lmat, Umat
 
 
 def _equation_coefficients(params):

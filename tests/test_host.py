@@ -1,6 +1,7 @@
 """Host-side logic that runs without a GPU: the C-ABI library's exports, the
-compact coefficient tables of setup(), setup()'s input validation, the grid
-builders, hs2_indexing, the launch-shape candidate lists and the rocprof
+compact coefficient tables of setup(), setup()'s input validation, the
+upstream's assembled operators and Thomas utilities, the module names, the
+grid builders, hs2_indexing, the launch-shape candidate lists and the rocprof
 summary reader."""
 import ctypes
 import os
@@ -10,7 +11,7 @@ import numpy as np
 import pytest
 
 import heatsim2_amd as hs
-from heatsim2_amd import _lib, autotune, crank_nicolson, hs2_indexing, rocprof_summary
+from heatsim2_amd import _lib, autotune, crank_nicolson, hs2_indexing, rocprof_summary, tridiag
 from hs2_cases import load_golden, to_package_args
 
 import oracle
@@ -172,14 +173,93 @@ def test_reference_operators_match_upstream(name):
             ref = sp.csr_matrix((z[key + "_data"], z[key + "_indices"], z[key + "_indptr"]),
                                 shape=(n, n))
             assert _rel_dense(mine.toarray(), ref.toarray()) < 1e-15, key
-        line = shape[(2, 1, 0)[s]]
-        L, U = crank_nicolson.tridiaglu(A, line)
+        L, U = tridiag.tridiaglu(A)
         assert (U[:, 1] == 1.0).all() and (U[:, 0] == 0.0).all() and (L[:, 2] == 0.0).all()
         # (L U)[r, r-1] = L[r, 0]; (L U)[r, r] = L[r, 0] U[r-1, 2] + L[r, 1];
         # (L U)[r, r+1] = L[r, 1] U[r, 2]
         prev = np.concatenate([[0.0], U[:-1, 2]])
         LU = np.stack([L[:, 0], L[:, 0] * prev + L[:, 1], L[:, 1] * U[:, 2]], axis=1)
         assert _rel_dense(LU, A) < 1e-14, s
+
+
+def _tridiag_system(rng, nlines, line, cuts=0.1):
+    """Diagonally dominant lines of `line` rows, a = 0 at each line's first
+    row and c = 0 at its last, plus some zero couplings inside lines."""
+    n = nlines * line
+    A = np.empty((n, 3))
+    A[:, 0] = -rng.random(n)
+    A[:, 2] = -rng.random(n)
+    A[:, 1] = 2.5 + rng.random(n)
+    A[::line, 0] = 0.0
+    A[line - 1::line, 2] = 0.0
+    A[rng.random(n) < cuts, 0] = 0.0
+    A[rng.random(n) < cuts, 2] = 0.0
+    return A
+
+
+def _dense(A):
+    n = A.shape[0]
+    return np.diag(A[:, 1]) + np.diag(A[1:, 0], -1) + np.diag(A[:-1, 2], 1)
+
+
+def test_tridiag_solves_and_factors():
+    """tridiag.tridiaglu / tridiagsolve (tridiag.pyx:9-69): L U equals A and
+    the solve equals a dense solve, on many lines with zero couplings inside
+    them; and the upstream's 4 x 4 diagnostic (tridiag.pyx:71-112), seeded."""
+    rng = np.random.default_rng(3)
+    for nlines, line in ((1, 1), (1, 7), (5, 16), (37, 33)):
+        A = _tridiag_system(rng, nlines, line)
+        L, U = tridiag.tridiaglu(A)
+        dense_L = np.diag(L[:, 1]) + np.diag(L[1:, 0], -1)
+        dense_U = np.diag(U[:, 1]) + np.diag(U[:-1, 2], 1)
+        assert np.abs(dense_L @ dense_U - _dense(A)).max() < 1e-13
+        b = rng.random(A.shape[0])
+        x = tridiag.tridiagsolve(L, U, b)
+        assert np.abs(x - np.linalg.solve(_dense(A), b)).max() < 1e-13 * np.abs(x).max()
+    A = rng.random((4, 3))
+    A[0, 0] = A[-1, 2] = 0.0
+    L, U = tridiag.tridiaglu(A)
+    dense_A = _dense(A)
+    LU = (np.diag(L[:, 1]) + np.diag(L[1:, 0], -1)) @ (np.diag(U[:, 1]) + np.diag(U[:-1, 2], 1))
+    assert np.linalg.norm(dense_A - LU) / np.linalg.norm(dense_A) < 1e-10
+    b = rng.random(4)
+    x = tridiag.tridiagsolve(L, U, b)
+    ref = np.linalg.inv(dense_A) @ b
+    assert np.linalg.norm(ref - x) / np.linalg.norm(ref) < 1e-10
+    with pytest.raises(AssertionError):
+        tridiag.tridiaglu(np.ones((3, 3)))
+
+
+def test_tridiag_runs_equal_one_recurrence():
+    """Splitting rows into runs at zero couplings gives bit for bit what one
+    recurrence over all rows gives (the upstream's serial pass)."""
+    rng = np.random.default_rng(4)
+    A = _tridiag_system(rng, 9, 20, cuts=0.2)
+    n = A.shape[0]
+    d = np.empty(n)
+    cp = np.empty(n)
+    for r in range(n):
+        d[r] = A[r, 1] if r == 0 else A[r, 1] - cp[r - 1] * A[r, 0]
+        cp[r] = A[r, 2] / d[r]
+    L, U = tridiag.tridiaglu(A)
+    assert np.array_equal(L[:, 1], d) and np.array_equal(U[:, 2], cp)
+    b = rng.random(n)
+    y = np.empty(n)
+    for r in range(n):
+        y[r] = b[r] / d[r] if r == 0 else (b[r] - A[r, 0] * y[r - 1]) / d[r]
+    for r in range(n - 2, -1, -1):
+        y[r] = y[r] - cp[r] * y[r + 1]
+    assert np.array_equal(tridiag.tridiagsolve(L, U, b), y)
+
+
+def test_upstream_module_names():
+    """The names heatsim2/__init__.py:15-17 imports exist under the same
+    module names."""
+    import heatsim2_amd.alternatingdirection_c_pyx as adc
+    assert adc.run_adi_steps is hs.run_adi_steps
+    assert hs.alternatingdirection.pyadi_step is crank_nicolson.adi_step
+    assert hs.alternatingdirection_c_pyx.adi_params is crank_nicolson.adi_params
+    assert hs.setup is crank_nicolson.setup
 
 
 def test_curvature_setup_arguments():
