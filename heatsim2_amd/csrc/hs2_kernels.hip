@@ -266,12 +266,25 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
             const double2 v1 = buf_load_d2(bT, o[k] * 8u + 16u);
             (*dst[k])[0] = v0.x; (*dst[k])[1] = v0.y; (*dst[k])[2] = v1.x; (*dst[k])[3] = v1.y;
           }
-          TL[b] = buf_load_d(bT, (unsigned)(b0 + xl) * 8u);
-          TR[b] = buf_load_d(bT, (unsigned)(b0 + xr) * 8u);
           const uint32_t wx = buf_load_u32(bC[2], o[0]);
           unpack4(wx, rx[b]);
           if constexpr (RY) rxw[it][b] = wx;
-          rxn[b] = buf_load_u8(bC[2], (unsigned)(b0 + xr));
+          if constexpr (FQ && SEG * G == 4 * kWave && NT % kWave == 0) {
+            // a wave holds one whole line, quad x0/4 in lane x0/4: the cells
+            // either side of the quad are in the neighbouring lanes (at the
+            // line's ends the cell itself, as the clamped loads give)
+            const int lane = xq[b];
+            const double l3 = __shfl_up(T[b][3], 1, kWave);
+            const double r0 = __shfl_down(T[b][0], 1, kWave);
+            const uint32_t rw = __shfl_down(wx, 1, kWave);
+            TL[b] = lane > 0 ? l3 : T[b][0];
+            TR[b] = lane < kWave - 1 ? r0 : T[b][3];
+            rxn[b] = lane < kWave - 1 ? (rw & 0xffu) : rx[b][3];
+          } else {
+            TL[b] = buf_load_d(bT, (unsigned)(b0 + xl) * 8u);
+            TR[b] = buf_load_d(bT, (unsigned)(b0 + xr) * 8u);
+            rxn[b] = buf_load_u8(bC[2], (unsigned)(b0 + xr));
+          }
           unpack4(buf_load_u32(bC[1], o[0]), ry[b]);
           unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
           unpack4(buf_load_u32(bC[0], o[0]), rz[b]);
