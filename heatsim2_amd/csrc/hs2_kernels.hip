@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
 // the class bytes and solves like k_sweep_x.  LPB lines per block.  SRC adds
 // the volumetric source terms (only on the rare steps that carry one).
 template <class V, int SEG, int G, int LPB, bool SRC, bool FULL = false, bool BUF = false,
-          bool RY = false>
+          bool RY = false, bool PZ = false>
 __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__restrict__ Tn,
                                               double *__restrict__ Ts,
                                               const uint8_t *__restrict__ vmap,
@@ -184,7 +184,19 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
     // as well as y ones (fewer rows fetched per line).
     int k, j;
     bool ok;
-    if (ty > 0) {
+    if (PZ) {
+      // 8 planes x 2 rows, numbered so that the two quads a thread holds in
+      // one phase-1 batch (lines l and l + 4, l = 8 * batch + 4 * b + wave)
+      // are planes k and k + 1 of one row: each is the other's z neighbour
+      const int nbj = (ny + 1) / 2;
+      const int kt = blk / nbj, jt = blk - kt * nbj;
+      const int it = tid / 8, b = (tid / 4) % 2, wv = tid % 4;
+      k = kt * 8 + (wv / 2) * 4 + it * 2 + b;
+      j = jt * 2 + wv % 2;
+      ok = k < nz && j < ny;
+      k = min(k, nz - 1);
+      j = min(j, ny - 1);
+    } else if (ty > 0) {
       const int tz = LPB / ty, nbj = (ny + ty - 1) / ty;
       const int kt = blk / nbj, jt = blk - kt * nbj;
       k = kt * tz + tid / ty;
@@ -223,6 +235,8 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   // T^n.  The batches are unrolled to keep those in registers.
   constexpr int NIT = FQ ? SEG / (4 * UQ) : 1;
   static_assert(!RY || (FQ && BUF), "RY: full-line class-map kernel");
+  static_assert(!PZ || (FQ && BUF && UQ == 2 && NT == 256 && SEG * G == 256 && LPB == 16),
+                "PZ: 16 x 16-cell lines, 16 lines per block");
   [[maybe_unused]] double hly[RY ? NIT : 1][UQ][4];
   [[maybe_unused]] uint32_t rxw[RY ? NIT : 1][UQ];
   if ((nx & 3) == 0) {
@@ -262,6 +276,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           double (*dst[5])[4] = {&T[b], &Tym[b], &Typ[b], &Tzm[b], &Tzp[b]};
 #pragma unroll
           for (int k = 0; k < 5; k++) {
+            if (PZ && ((b == 0 && k == 4) || (b == 1 && k == 3))) continue;  // set below
             const double2 v0 = buf_load_d2(bT, o[k] * 8u);
             const double2 v1 = buf_load_d2(bT, o[k] * 8u + 16u);
             (*dst[k])[0] = v0.x; (*dst[k])[1] = v0.y; (*dst[k])[2] = v1.x; (*dst[k])[3] = v1.y;
@@ -288,7 +303,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           unpack4(buf_load_u32(bC[1], o[0]), ry[b]);
           unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
           unpack4(buf_load_u32(bC[0], o[0]), rz[b]);
-          unpack4(buf_load_u32(bC[0], o[4]), rzp[b]);
+          if (!(PZ && b == 0)) unpack4(buf_load_u32(bC[0], o[4]), rzp[b]);
         } else {
         const double2 *p0 = reinterpret_cast<const double2 *>(Tn + b0 + x0);
         const double2 *p1 = reinterpret_cast<const double2 *>(Tn + lbase[lb][1] + x0);
@@ -317,6 +332,14 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           view.fetch4(1, lbase[lb][2] + x0, ryp[b]);
           view.fetch4(0, lbase[lb][4] + x0, rzp[b]);
         }
+        }
+      }
+      if constexpr (PZ) {  // quad 1 sits one plane above quad 0
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          Tzp[0][q] = T[1][q];
+          Tzm[1][q] = T[0][q];
+          rzp[0][q] = rz[1][q];
         }
       }
 #pragma unroll
@@ -571,7 +594,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   }
 }
 
-template <class V, int SEG, int G, int LPB, bool SRC, bool FULL, bool RY = false>
+template <class V, int SEG, int G, int LPB, bool SRC, bool FULL, bool RY = false, bool PZ = false>
 __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
                                                     const double *__restrict__ Tn,
                                                     double *__restrict__ Ts,
@@ -579,8 +602,8 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
                                                     const double *__restrict__ vtab_g,
                                                     const double *__restrict__ sfield, int ty) {
   constexpr bool BUF = FULL && SEG % 4 == 0 && std::is_same<V, ClassView>::value;
-  sweep_x2_body<V, SEG, G, LPB, SRC, FULL, BUF, RY>(g, view, Tn, Ts, vmap, vtab_g, sfield,
-                                                    xcd_remap(blockIdx.x, gridDim.x), ty);
+  sweep_x2_body<V, SEG, G, LPB, SRC, FULL, BUF, RY, PZ>(g, view, Tn, Ts, vmap, vtab_g, sfield,
+                                                        xcd_remap(blockIdx.x, gridDim.x), ty);
 }
 
 // ---------------------------------------------------------- y / z sweeps
@@ -790,9 +813,20 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *T
       sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + (src ? kMaxVtab : 0) + LPB * RS);
   const bool full = x2_full(g, SEG * G, LPB, ty);
   if constexpr (std::is_same<V, ClassView>::value && SEG % 4 == 0) {
+    // 16 x 16-cell lines in 8 x 2 tiles: the plane-paired line order
+    constexpr bool kPZ = SEG == 16 && G == 16 && LPB == 16;
     if (ry) {  // the caller checked x_ry_ok and that the step has no sources
       if (!full || src) return hipErrorInvalidValue;
-      hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, true, true>), dim3(blocks),
+      if (kPZ && ty == 2)
+        hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, true, true, kPZ>), dim3(blocks),
+                           dim3(G * LPB), shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
+      else
+        hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, true, true>), dim3(blocks),
+                           dim3(G * LPB), shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
+      return hipGetLastError();
+    }
+    if (kPZ && ty == 2 && full && !src) {
+      hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, true, false, kPZ>), dim3(blocks),
                          dim3(G * LPB), shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
       return hipGetLastError();
     }

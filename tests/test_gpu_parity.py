@@ -247,6 +247,35 @@ def test_y_from_x_path_matches_oracle(hs):
             assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, (x, n)
 
 
+def test_x_plane_pairs_match_oracle(hs):
+    """16 x 16-cell x lines in 8 x 2 tiles run with the lines ordered so that
+    each thread's two quads are vertically adjacent and share their z rows.
+    Steps 0-3 carry sources (T* path), steps 4-5 do not (R_y path).  Against
+    the oracle step by step, and bit for bit against 1-row tiles, which take
+    the same arithmetic per cell."""
+    from heatsim2_amd import _lib, autotune
+    case = _random_case((16, 10, 256), seed=23)
+    case["nsteps"] = 6
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    ref = oracle.run_ref(case)
+    rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
+    out = {}
+    for x in ((16, 16, 16, 2), (16, 16, 16, 1)):
+        autotune._apply(_lib.load(), P.plan, {"x": x, "y": (16, 8), "z": (16, 8)})
+        assert _lib.plan_info(P.plan)["y_from_x"] == 1
+        T = torch.from_numpy(case["T0"]).cuda()
+        steps = []
+        for n in range(case["nsteps"]):
+            T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"],
+                                 args["volumetric"])
+            steps.append(T.cpu().numpy())
+            assert _rel(steps[-1], ref[n], rise) < TOL, (x, n)
+        out[x] = steps
+    for a, b in zip(out[(16, 16, 16, 2)], out[(16, 16, 16, 1)]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("name", ["tiny_mixed", "tiny_aniso", "tiny_curved"])
 def test_step_operators_match_upstream(hs, name):
     """ADI_steps[s].Amat/Bmat/Cmats/Dvec of a device plan (per-cell
