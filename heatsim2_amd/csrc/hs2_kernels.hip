@@ -239,6 +239,12 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
                 "PZ: 16 x 16-cell lines, 16 lines per block");
   [[maybe_unused]] double hly[RY ? NIT : 1][UQ][4];
   [[maybe_unused]] uint32_t rxw[RY ? NIT : 1][UQ];
+  // PZ with RY: a thread's four quads are planes k..k+3 of one row over the
+  // two unrolled batches; the second batch's first quad (plane k+2) and its
+  // z-1 row (plane k+1) were already loaded by the first batch
+  constexpr bool PZ4 = PZ && RY && NIT == 2;
+  [[maybe_unused]] double cT1[4], cT2[4];
+  [[maybe_unused]] typename V::Raw cRz2[4];
   if ((nx & 3) == 0) {
     const int qpl = FQ ? SEG * G / 4 : nx >> 2;
     const int total = LPB * qpl;
@@ -268,6 +274,16 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
         const int x0 = xq[b] << 2;
         const int b0 = lbase[lb][0];
         const int xl = x0 > 0 ? x0 - 1 : 0, xr = x0 + 4 < nx ? x0 + 4 : nx - 1;
+        if constexpr (PZ4) {
+          if (it == 1 && b == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              T[0][q] = cT2[q];
+              Tzm[0][q] = cT1[q];
+              rz[0][q] = cRz2[q];
+            }
+          }
+        }
         if constexpr (BUF) {
           // 32-bit byte offsets from buffer descriptors: no 64-bit address math
           const unsigned o[5] = {(unsigned)(b0 + x0), (unsigned)(lbase[lb][1] + x0),
@@ -277,6 +293,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
 #pragma unroll
           for (int k = 0; k < 5; k++) {
             if (PZ && ((b == 0 && k == 4) || (b == 1 && k == 3))) continue;  // set below
+            if (PZ4 && it == 1 && b == 0 && (k == 0 || k == 3)) continue;   // carried
             const double2 v0 = buf_load_d2(bT, o[k] * 8u);
             const double2 v1 = buf_load_d2(bT, o[k] * 8u + 16u);
             (*dst[k])[0] = v0.x; (*dst[k])[1] = v0.y; (*dst[k])[2] = v1.x; (*dst[k])[3] = v1.y;
@@ -302,7 +319,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           }
           unpack4(buf_load_u32(bC[1], o[0]), ry[b]);
           unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
-          unpack4(buf_load_u32(bC[0], o[0]), rz[b]);
+          if (!(PZ4 && it == 1 && b == 0)) unpack4(buf_load_u32(bC[0], o[0]), rz[b]);
           if (!(PZ && b == 0)) unpack4(buf_load_u32(bC[0], o[4]), rzp[b]);
         } else {
         const double2 *p0 = reinterpret_cast<const double2 *>(Tn + b0 + x0);
@@ -340,6 +357,16 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           Tzp[0][q] = T[1][q];
           Tzm[1][q] = T[0][q];
           rzp[0][q] = rz[1][q];
+        }
+      }
+      if constexpr (PZ4) {  // planes k+1 and k+2 for the second batch
+        if (it == 0) {
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            cT1[q] = T[1][q];
+            cT2[q] = Tzp[1][q];
+            cRz2[q] = rzp[1][q];
+          }
         }
       }
 #pragma unroll
