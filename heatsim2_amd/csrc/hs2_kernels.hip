@@ -187,12 +187,13 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
     if (PZ) {
       // 8 planes x 2 rows, numbered so that the two quads a thread holds in
       // one phase-1 batch (lines l and l + 4, l = 8 * batch + 4 * b + wave)
-      // are planes k and k + 1 of one row: each is the other's z neighbour
+      // are rows j and j + 1 of one plane (each is the other's y neighbour),
+      // and its next batch holds the same two rows one plane up
       const int nbj = (ny + 1) / 2;
       const int kt = blk / nbj, jt = blk - kt * nbj;
       const int it = tid / 8, b = (tid / 4) % 2, wv = tid % 4;
-      k = kt * 8 + (wv / 2) * 4 + it * 2 + b;
-      j = jt * 2 + wv % 2;
+      k = kt * 8 + wv * 2 + it;
+      j = jt * 2 + b;
       ok = k < nz && j < ny;
       k = min(k, nz - 1);
       j = min(j, ny - 1);
@@ -239,12 +240,12 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
                 "PZ: 16 x 16-cell lines, 16 lines per block");
   [[maybe_unused]] double hly[RY ? NIT : 1][UQ][4];
   [[maybe_unused]] uint32_t rxw[RY ? NIT : 1][UQ];
-  // PZ with RY: a thread's four quads are planes k..k+3 of one row over the
-  // two unrolled batches; the second batch's first quad (plane k+2) and its
-  // z-1 row (plane k+1) were already loaded by the first batch
+  // PZ with RY: a thread's four quads are rows j, j+1 of planes k, k+1 over
+  // the two unrolled batches; the first batch's z+1 rows are the second's own
+  // rows and its own rows the second's z-1 rows, so they are carried
   constexpr bool PZ4 = PZ && RY && NIT == 2;
-  [[maybe_unused]] double cT1[4], cT2[4];
-  [[maybe_unused]] typename V::Raw cRz2[4];
+  [[maybe_unused]] double cT1[2][4], cT2[2][4];
+  [[maybe_unused]] typename V::Raw cRz2[2][4];
   if ((nx & 3) == 0) {
     const int qpl = FQ ? SEG * G / 4 : nx >> 2;
     const int total = LPB * qpl;
@@ -275,12 +276,12 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
         const int b0 = lbase[lb][0];
         const int xl = x0 > 0 ? x0 - 1 : 0, xr = x0 + 4 < nx ? x0 + 4 : nx - 1;
         if constexpr (PZ4) {
-          if (it == 1 && b == 0) {
+          if (it == 1) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-              T[0][q] = cT2[q];
-              Tzm[0][q] = cT1[q];
-              rz[0][q] = cRz2[q];
+              T[b][q] = cT2[b][q];
+              Tzm[b][q] = cT1[b][q];
+              rz[b][q] = cRz2[b][q];
             }
           }
         }
@@ -292,8 +293,8 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           double (*dst[5])[4] = {&T[b], &Tym[b], &Typ[b], &Tzm[b], &Tzp[b]};
 #pragma unroll
           for (int k = 0; k < 5; k++) {
-            if (PZ && ((b == 0 && k == 4) || (b == 1 && k == 3))) continue;  // set below
-            if (PZ4 && it == 1 && b == 0 && (k == 0 || k == 3)) continue;   // carried
+            if (PZ && ((b == 0 && k == 2) || (b == 1 && k == 1))) continue;  // set below
+            if (PZ4 && it == 1 && (k == 0 || k == 3)) continue;              // carried
             const double2 v0 = buf_load_d2(bT, o[k] * 8u);
             const double2 v1 = buf_load_d2(bT, o[k] * 8u + 16u);
             (*dst[k])[0] = v0.x; (*dst[k])[1] = v0.y; (*dst[k])[2] = v1.x; (*dst[k])[3] = v1.y;
@@ -318,9 +319,9 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
             rxn[b] = buf_load_u8(bC[2], (unsigned)(b0 + xr));
           }
           unpack4(buf_load_u32(bC[1], o[0]), ry[b]);
-          unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
-          if (!(PZ4 && it == 1 && b == 0)) unpack4(buf_load_u32(bC[0], o[0]), rz[b]);
-          if (!(PZ && b == 0)) unpack4(buf_load_u32(bC[0], o[4]), rzp[b]);
+          if (!(PZ && b == 0)) unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
+          if (!(PZ4 && it == 1)) unpack4(buf_load_u32(bC[0], o[0]), rz[b]);
+          unpack4(buf_load_u32(bC[0], o[4]), rzp[b]);
         } else {
         const double2 *p0 = reinterpret_cast<const double2 *>(Tn + b0 + x0);
         const double2 *p1 = reinterpret_cast<const double2 *>(Tn + lbase[lb][1] + x0);
@@ -351,22 +352,24 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
         }
         }
       }
-      if constexpr (PZ) {  // quad 1 sits one plane above quad 0
+      if constexpr (PZ) {  // quad 1 is the row after quad 0
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          Tzp[0][q] = T[1][q];
-          Tzm[1][q] = T[0][q];
-          rzp[0][q] = rz[1][q];
+          Typ[0][q] = T[1][q];
+          Tym[1][q] = T[0][q];
+          ryp[0][q] = ry[1][q];
         }
       }
-      if constexpr (PZ4) {  // planes k+1 and k+2 for the second batch
+      if constexpr (PZ4) {  // the second batch is the plane above
         if (it == 0) {
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
-            cT1[q] = T[1][q];
-            cT2[q] = Tzp[1][q];
-            cRz2[q] = rzp[1][q];
-          }
+          for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              cT1[b][q] = T[b][q];
+              cT2[b][q] = Tzp[b][q];
+              cRz2[b][q] = rzp[b][q];
+            }
         }
       }
 #pragma unroll
