@@ -162,6 +162,9 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   const int tabpad = (view.lds_doubles(-1) + 1) & ~1;
   double *vtab = lds + tabpad;
   double *rbuf = vtab + (SRC ? kMaxVtab : 0);
+  // BUF: phase 1's x-class words, one per quad, for phase 2 (which would
+  // otherwise fetch them again right after the barrier)
+  [[maybe_unused]] uint32_t *cbuf = reinterpret_cast<uint32_t *>(rbuf + LPB * RS);
   if (SRC && vmap)
     for (int t = threadIdx.x; t < kMaxVtab; t += NT) vtab[t] = vtab_g[t];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -302,6 +305,7 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           const uint32_t wx = buf_load_u32(bC[2], o[0]);
           unpack4(wx, rx[b]);
           if constexpr (RY) rxw[it][b] = wx;
+          cbuf[li[b] * (SEG * G / 4) + xq[b]] = wx;
           if constexpr (FQ && SEG * G == 4 * kWave && NT % kWave == 0) {
             // a wave holds one whole line, quad x0/4 in lane x0/4: the cells
             // either side of the quad are in the neighbouring lanes (at the
@@ -491,11 +495,12 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   const int i0 = lig * SEG;
   typename V::Raw rxs[SEG];
   if constexpr (BUF) {
-    static_assert(SEG % 4 == 0, "BUF path loads class words");
+    static_assert(SEG % 4 == 0, "BUF path reads class words");
+    const uint32_t *cw = cbuf + lq * (SEG * G / 4) + i0 / 4;
 #pragma unroll
     for (int q = 0; q < SEG; q += 4) {
       typename V::Raw t4[4];
-      unpack4(buf_load_u32(bC[2], (unsigned)(base + i0 + q)), t4);
+      unpack4(cw[q / 4], t4);
 #pragma unroll
       for (int u = 0; u < 4; u++) rxs[q + u] = t4[u];
     }
@@ -839,8 +844,10 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *T
                             : (nlines + LPB - 1) / LPB;
   constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;
   const bool src = vmap || sfield;
+  // BUF variants (class maps, SEG % 4 == 0) also park one class word per quad
   const size_t shm =
-      sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + (src ? kMaxVtab : 0) + LPB * RS);
+      sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + (src ? kMaxVtab : 0) + LPB * RS) +
+      (std::is_same<V, ClassView>::value && SEG % 4 == 0 ? (size_t)LPB * SEG * G : 0);
   const bool full = x2_full(g, SEG * G, LPB, ty);
   if constexpr (std::is_same<V, ClassView>::value && SEG % 4 == 0) {
     // 16 x 16-cell lines in 8 x 2 tiles: the plane-paired line order
