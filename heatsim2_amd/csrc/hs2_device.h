@@ -304,6 +304,79 @@ __device__ __forceinline__ void segment_reduce(double (&a)[N], double (&c)[N], d
   }
 }
 
+// segment_reduce on rows O..O+M-1 of the arrays, in place (same algebra,
+// indices shifted): leaves the end rows of that sub-segment and its interior
+// rows in terms of x_O and x_{O+M-1}.
+template <int N, int O, int M>
+__device__ __forceinline__ void sub_reduce(double (&a)[N], double (&c)[N], double (&r)[N],
+                                           double &As, double &Cs, double &Rs, double &Ae,
+                                           double &Ce, double &Re) {
+  static_assert(M >= 3 && O + M <= N, "sub-segment of three rows or more");
+#pragma unroll
+  for (int q = O + 2; q < O + M; q++) {
+    const double k = 1.0 / (1.0 - a[q] * c[q - 1]);
+    const double an = -a[q] * a[q - 1] * k;
+    c[q] = c[q] * k;
+    r[q] = (r[q] - a[q] * r[q - 1]) * k;
+    a[q] = an;
+  }
+  Ae = a[O + M - 1];
+  Ce = c[O + M - 1];
+  Re = r[O + M - 1];
+#pragma unroll
+  for (int q = O + M - 3; q >= O + 1; q--) {
+    a[q] = a[q] - c[q] * a[q + 1];
+    r[q] = r[q] - c[q] * r[q + 1];
+    c[q] = -c[q] * c[q + 1];
+  }
+  const double k = 1.0 / (1.0 - c[O] * a[O + 1]);
+  As = a[O] * k;
+  Cs = -c[O] * c[O + 1] * k;
+  Rs = (r[O] - c[O] * r[O + 1]) * k;
+}
+
+// The two inner end cells of a segment reduced in halves, as affine
+// functions of the segment's end cells: x_{h-1} = P1 + A1 x_0 + C1 x_{N-1},
+// x_h = P2 + A2 x_0 + C2 x_{N-1} (h = N/2).
+struct HalfJoin {
+  double P1, A1, C1, P2, A2, C2;
+};
+
+// segment_reduce by halves: rows 0..h-1 and h..N-1 are reduced
+// independently (two dependency chains instead of one), then joined.  The
+// join eliminates x_{h-1} and x_h between the halves' four end rows
+//   As1 x_b + x_0 + Cs1 x_{h-1} = Rs1,   Ae1 x_0 + x_{h-1} + Ce1 x_h = Re1,
+//   As2 x_{h-1} + x_h + Cs2 x_{N-1} = Rs2,   Ae2 x_h + x_{N-1} + Ce2 x_a = Re2,
+// leaving the same two end rows as segment_reduce; interior rows stay in
+// terms of their half's ends (x_0, x_{h-1}) and (x_h, x_{N-1}).
+template <int N>
+__device__ __forceinline__ void segment_reduce_halves(double (&a)[N], double (&c)[N],
+                                                      double (&r)[N], double &As, double &Cs,
+                                                      double &Rs, double &Ae, double &Ce,
+                                                      double &Re, HalfJoin &j) {
+  static_assert(N % 2 == 0 && N >= 6, "two halves of three rows or more");
+  constexpr int H = N / 2;
+  double As1, Cs1, Rs1, Ae1, Ce1, Re1, As2, Cs2, Rs2, Ae2, Ce2, Re2;
+  sub_reduce<N, 0, H>(a, c, r, As1, Cs1, Rs1, Ae1, Ce1, Re1);
+  sub_reduce<N, H, H>(a, c, r, As2, Cs2, Rs2, Ae2, Ce2, Re2);
+  // x_{h-1} + Ce1 x_h = Re1 - Ae1 x_0 and As2 x_{h-1} + x_h = Rs2 - Cs2 x_{N-1}
+  const double k = 1.0 / (1.0 - Ce1 * As2);
+  j.P1 = (Re1 - Ce1 * Rs2) * k;
+  j.A1 = -Ae1 * k;
+  j.C1 = Ce1 * Cs2 * k;
+  j.P2 = (Rs2 - As2 * Re1) * k;
+  j.A2 = As2 * Ae1 * k;
+  j.C2 = -Cs2 * k;
+  const double d0 = 1.0 / (1.0 + Cs1 * j.A1);
+  As = As1 * d0;
+  Cs = Cs1 * j.C1 * d0;
+  Rs = (Rs1 - Cs1 * j.P1) * d0;
+  const double d1 = 1.0 / (1.0 + Ae2 * j.C2);
+  Ae = Ae2 * j.A2 * d1;
+  Ce = Ce2 * d1;
+  Re = (Re2 - Ae2 * j.P2) * d1;
+}
+
 // Segment elimination, streamed row by row.  Rows q = 0..N-1 of one
 // contiguous segment, a_q x_{q-1} + b_q x_q + c_q x_{q+1} = r_q (b_q > 0),
 // are fed in order with row(q, ...); one reciprocal per row.  finish() then

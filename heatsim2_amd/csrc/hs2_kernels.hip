@@ -728,8 +728,14 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
     }
     en = ep;
   }
+  // 16-cell segments are reduced by halves: two shorter division chains
+  constexpr bool HALVES = SEG == 16;
   double As, Cs, Rs, Ae, Ce, Re;
-  segment_reduce<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
+  [[maybe_unused]] HalfJoin hj;
+  if constexpr (HALVES)
+    segment_reduce_halves<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re, hj);
+  else
+    segment_reduce<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
   ra[(2 * p) * CW + cl] = As; rc[(2 * p) * CW + cl] = Cs; rr[(2 * p) * CW + cl] = Rs;
   ra[(2 * p + 1) * CW + cl] = Ae; rc[(2 * p + 1) * CW + cl] = Ce; rr[(2 * p + 1) * CW + cl] = Re;
   __syncthreads();
@@ -739,8 +745,20 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
   double x[SEG];
   x[0] = xs;
   x[SEG - 1] = xe;
+  if constexpr (HALVES) {
+    constexpr int H = SEG / 2;
+    const double x1 = hj.P1 + hj.A1 * xs + hj.C1 * xe;
+    const double x2 = hj.P2 + hj.A2 * xs + hj.C2 * xe;
+    x[H - 1] = x1;
+    x[H] = x2;
 #pragma unroll
-  for (int q = 1; q < SEG - 1; q++) x[q] = r[q] - a[q] * xs - c[q] * xe;
+    for (int q = 1; q < H - 1; q++) x[q] = r[q] - a[q] * xs - c[q] * x1;
+#pragma unroll
+    for (int q = H + 1; q < SEG - 1; q++) x[q] = r[q] - a[q] * x2 - c[q] * xe;
+  } else {
+#pragma unroll
+    for (int q = 1; q < SEG - 1; q++) x[q] = r[q] - a[q] * xs - c[q] * xe;
+  }
 #pragma unroll
   for (int q = 0; q < SEG; q++) {
     const int t = t0 + q;
