@@ -432,7 +432,7 @@ extern "C" {
 
 const char *hs2_last_error(void) { return g_last_error.c_str(); }
 
-const char *hs2_version(void) { return "heatsim2_amd 0.3 (gfx950, fp64)"; }
+const char *hs2_version(void) { return "heatsim2_amd 0.4 (gfx950, fp64)"; }
 
 int hs2_plan_create(const hs2_grid_desc *grid, hs2_plan **out) {
   if (!grid || !out) return fail(HS2_E_ARG, "hs2_plan_create: null argument");
