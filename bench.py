@@ -22,12 +22,16 @@ steps run right after the timed region (hs2_run_timed; each event pair adds a
 few microseconds of its own, so the achieved rate is slightly understated);
 peak = 8 TB/s.  traffic: the HBM bytes per launch of that kernel read from
 the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
-(profiles/hbm_traffic_two_layer_256.txt, copy-calibrated).
+(profiles/hbm_traffic_two_layer_256.txt, copy-calibrated).  copy_GBps: a
+device-to-device copy of one field on the same box (read + write bytes per
+copy, events on torch's stream); step_frac_of_copy puts the step's canonical
+64 B per cell-step against it.
 
 cpu_baseline: the oracle's reference-structured step (assembled permuted
 CSR operators, global Thomas with precomputed LU, transposes -- the
 upstream's structure) on one host thread, over a few steps of the same
-256^3 workload, reported as cell-steps/s.
+256^3 workload, reported as cell-steps/s, with the host CPUs available to
+the process beside the one it uses.
 """
 import argparse
 import json
@@ -52,6 +56,27 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def device_copy_gbs(T, reps=20):
+    """Bandwidth of a device-to-device copy of one field (bytes read plus
+    bytes written per copy), timed with events on torch's stream."""
+    import torch
+    a = torch.empty_like(T)
+    a.copy_(T)
+    b = torch.empty_like(T)
+    b.copy_(a)  # warm
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 2 * a.numel() * a.element_size()
+    del a, b
+    return nbytes / (ms * 1e-3) / 1e9
+
+
 def cpu_baseline(case, nsteps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -67,6 +92,7 @@ def cpu_baseline(case, nsteps):
     dt = time.time() - t0
     del plan
     return dict(value=n * nsteps / dt, unit="cell-steps/s", cores=1, kind="port",
+                host_cpus_available=len(os.sched_getaffinity(0)), host_cpus=os.cpu_count(),
                 sample="%d source-free steps of the same %dx%dx%d two-layer workload through the "
                        "oracle's reference-structured step (permuted CSR B/C operators, global "
                        "Thomas with precomputed LU, transposes), one thread; %.1f s timed, "
@@ -348,6 +374,11 @@ def main():
         log("  %s: %.4f ms/launch, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
             % (names[i], kms[i], gbs[i], 100 * gbs[i] / HBM_PEAK_GBS))
     step_gbs = CANONICAL_STEP_BYTES * value / 1e9
+    # a plain device copy of one field on the same box, as a practical
+    # ceiling next to the 8 TB/s peak (read + write bytes per copy)
+    copy_gbs = device_copy_gbs(T)
+    log("device copy: %.0f GB/s; the step reaches %.1f%% of it on 64 B per cell-step"
+        % (copy_gbs, 100 * step_gbs / copy_gbs))
     # measured HBM bytes per launch: committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # summary of this workload (copy-calibrated), when it is for this size
     from heatsim2_amd.rocprof_summary import read_traffic
@@ -362,7 +393,8 @@ def main():
                     kernel_ms=[round(x, 5) for x in kms],
                     step_achieved=round(step_gbs, 1), step_frac=round(step_gbs / HBM_PEAK_GBS, 4),
                     step_bytes_per_cell=sum(bpc),
-                    step_frac_own_bytes=round(sum(bpc) * value / 1e9 / HBM_PEAK_GBS, 4))
+                    step_frac_own_bytes=round(sum(bpc) * value / 1e9 / HBM_PEAK_GBS, 4),
+                    copy_GBps=round(copy_gbs, 1), step_frac_of_copy=round(step_gbs / copy_gbs, 4))
 
     out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": K,
            "warmup": args.warmup, "ms_per_step": 1e3 * wall / K, "higher_is_better": True,
