@@ -868,7 +868,9 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *T
       (std::is_same<V, ClassView>::value && SEG % 4 == 0 ? (size_t)LPB * SEG * G : 0);
   const bool full = x2_full(g, SEG * G, LPB, ty);
   if constexpr (std::is_same<V, ClassView>::value && SEG % 4 == 0) {
-    // 16 x 16-cell lines in 8 x 2 tiles: the plane-paired line order
+    // 16 x 16-cell lines in 8 x 2 tiles: the paired line order (source-free
+    // steps on full lines always take this R_y path: x_ry_ok holds exactly
+    // when the lines are full)
     constexpr bool kPZ = SEG == 16 && G == 16 && LPB == 16;
     if (ry) {  // the caller checked x_ry_ok and that the step has no sources
       if (!full || src) return hipErrorInvalidValue;
@@ -878,11 +880,6 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *T
       else
         hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, true, true>), dim3(blocks),
                            dim3(G * LPB), shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
-      return hipGetLastError();
-    }
-    if (kPZ && ty == 2 && full && !src) {
-      hipLaunchKernelGGL((k_sweep_x2<V, SEG, G, LPB, false, true, false, kPZ>), dim3(blocks),
-                         dim3(G * LPB), shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
       return hipGetLastError();
     }
   }

@@ -249,10 +249,10 @@ def test_y_from_x_path_matches_oracle(hs):
 
 def test_x_plane_pairs_match_oracle(hs):
     """16 x 16-cell x lines in 8 x 2 tiles run with the lines ordered so that
-    each thread's quads are neighbours in y and z and share those rows.
-    Steps 0-3 carry sources (T* path), steps 4-5 do not (R_y path).  Against
-    the oracle step by step, and bit for bit against 1-row tiles, which take
-    the same arithmetic per cell."""
+    each thread's quads are neighbours in y and z and share those rows
+    (source-free steps 4-5; steps 0-3 carry sources and run the source
+    kernel).  Against the oracle step by step, and bit for bit against
+    1-row tiles, which take the same arithmetic per cell."""
     from heatsim2_amd import _lib, autotune
     case = _random_case((16, 10, 256), seed=23)
     case["nsteps"] = 6
