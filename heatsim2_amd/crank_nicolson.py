@@ -15,7 +15,9 @@ version:
 2. M = rho*c/dt per COMPUTE material (crank_nicolson.pyx:378), M = 1 and
    no coupling for FIXED (crank_nicolson.pyx:473-486);
 3. hands the uint8 maps (kept on the device) and these small tables to the
-   HIP library, whose sweep kernels derive every coefficient on the fly.
+   HIP library, whose sweep kernels derive every coefficient on the fly
+   (where an axis has more than 256 distinct coefficient classes -- many
+   materials -- the library keeps per-cell float64 coefficients instead).
 
 Validation happens here, before any launch, with the upstream's failure
 modes turned into exceptions: IndexError for map entries beyond the tables,
@@ -42,8 +44,6 @@ _T_ARGS = (
      "T46p", "T64m", "T64p"),
 )
 _AXIS_NAME = ("z", "y", "x")
-# compact tables must fit the kernels' LDS budget (doubles)
-MAX_TABLE_DOUBLES = 4096
 
 
 class UnalignedAnisotropicError(NotImplementedError, AssertionError):
@@ -561,10 +561,6 @@ def setup(z0, y0, x0,
     if curved:
         return _setup_curved(d, float(dt), (nz, ny, nx), materials, boundaries, mat, bz, by, bx,
                              mtab, compute, top_surface_y_curvatures, top_surface_x_curvatures)
-    if gtab.size + 2 * nmat > MAX_TABLE_DOUBLES:
-        raise NotImplementedError(
-            "%d materials x %d boundary classes exceed the compact coefficient table (%d "
-            "doubles)" % (nmat, nb, MAX_TABLE_DOUBLES))
     L = _lib.load()
     dev = _lib.device()
     params = adi_params(shape=(nz, ny, nx), volume_array=dz * dy * dx, d=d, dt=float(dt),

@@ -303,6 +303,43 @@ def test_step_operators_match_upstream(hs, name):
         assert _rel(LU, st.Amat, np.abs(st.Amat).max()) < 1e-14, s
 
 
+def test_many_classes_use_per_cell_coefficients(hs):
+    """40 materials of distinct k and rho*c, placed at random, give more than
+    256 coefficient classes on an axis: setup falls back to per-cell float64
+    coefficients (plan mode 2).  Trajectory against the oracle, and the
+    step operators against the oracle's reference-structured assembly."""
+    import scipy.sparse as sp
+    from heatsim2_amd import _lib
+    case = _random_case((6, 12, 14), seed=31)
+    rng = np.random.default_rng(32)
+    mats = [(0,) + (float(k),) * 3 + (float(r), float(c), 0)
+            for k, r, c in zip(rng.uniform(0.5, 60.0, 40), rng.uniform(1e3, 8e3, 40),
+                               rng.uniform(300.0, 1500.0, 40))]
+    case["materials"] = mats + [(1, 0, 0, 0, 0, 0, 0)]
+    m = rng.integers(0, 40, size=case["shape"]).astype(np.uint8)
+    m[rng.random(case["shape"]) < 0.05] = 40  # FIXED
+    case["maps"] = (m,) + tuple(case["maps"][1:])
+    case["T0"] = np.where(m == 40, 290.0, case["T0"])
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    assert _lib.plan_info(P.plan)["mode"] == 2
+    ref = oracle.run_ref(case)
+    rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
+    T = torch.from_numpy(case["T0"]).cuda()
+    for n in range(case["nsteps"]):
+        T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"], args["volumetric"])
+        assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, n
+    R = oracle.RefPlan(case["shape"], oracle.assemble_case(case))
+    n = R.n
+    for s, st in enumerate(S):
+        A, D = R.A(s)
+        assert _rel(st.Amat, A, np.abs(A).max()) < 1e-14, s
+        assert _rel(st.Dvec, D, 1.0) < 1e-14, s
+        for which, mine in enumerate([st.Bmat] + list(st.Cmats)):
+            want = sp.csr_matrix(R.csr(s, which), shape=(n, n)).toarray()
+            assert _rel(mine.toarray(), want, np.abs(want).max()) < 1e-14, (s, which)
+
+
 def test_run_many_equals_stepwise(hs):
     case = load_golden("tiny_mixed")
     args = to_package_args(hs, case)
