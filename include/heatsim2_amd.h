@@ -45,7 +45,9 @@ typedef struct {
 /* Create a plan for one grid.
  * Replaces adi_setup + pyadi_step.__init__ / create_adi_step
  * (alternatingdirection_c_pyx.pyx:418-445, :137-167;
- *  alternatingdirection_c.h:51). */
+ *  alternatingdirection_c.h:51).  Every axis length must be <= 1024 and the
+ * face-map sizes below 2^31 (HS2_E_UNSUPPORTED otherwise): the line kernels
+ * hold a line in registers and LDS, and index with 32 bits. */
 int hs2_plan_create(const hs2_grid_desc *grid, hs2_plan **out);
 
 /* Release a plan.  Replaces delete_adi_step (alternatingdirection_c.h:49,
