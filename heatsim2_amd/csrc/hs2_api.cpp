@@ -88,17 +88,19 @@ struct hs2_batch {
 
 namespace {
 
+// Untuned y / z shape: segments of up to 16 cells (32-cell segments spill
+// registers), and blocks of about 256 threads, as the tuner picks at 256 and
+// 512 cells per line.
 int default_strided(int L, hs2::SConfig *cfg) {
   int s;
   if (L <= 4) s = 2;
   else if (L <= 16) s = 4;
   else if (L <= 64) s = 8;
-  else if (L <= 256) s = 16;
-  else if (L <= 1024) s = 32;
+  else if (L <= 1024) s = 16;
   else return 0;
   const int P = (L + s - 1) / s;
   int w = 64;
-  while (w > 8 && P * w > kMaxBlockThreads) w >>= 1;
+  while (w > 8 && P * w > 256) w >>= 1;
   if (P * w > kMaxBlockThreads) return 0;
   *cfg = {s, w};
   return 1;
