@@ -188,14 +188,18 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
     int k, j;
     bool ok;
     if (PZ) {
-      // 8 planes x 2 rows, numbered so that the two quads a thread holds in
-      // one phase-1 batch (lines l and l + 4, l = 8 * batch + 4 * b + wave)
+      // 2 W planes x 2 rows, numbered so that the two quads a thread holds
+      // in one phase-1 batch (lines l and l + W, l = 2 W batch + W b + slab)
       // are rows j and j + 1 of one plane (each is the other's y neighbour),
       // and its next batch holds the same two rows one plane up
+      // W = lines per quad slab (NT threads over lines of SEG*G/4 quads):
+      // 4 for 16 x 16-cell lines in 16-line blocks, 1 for 16 x 32-cell lines
+      // in 4-line blocks; a tile is 2 W planes x 2 rows
+      constexpr int W = NT / (SEG * G / 4);
       const int nbj = (ny + 1) / 2;
       const int kt = blk / nbj, jt = blk - kt * nbj;
-      const int it = tid / 8, b = (tid / 4) % 2, wv = tid % 4;
-      k = kt * 8 + wv * 2 + it;
+      const int it = tid / (2 * W), b = (tid / W) % 2, wv = tid % W;
+      k = kt * 2 * W + wv * 2 + it;
       j = jt * 2 + b;
       ok = k < nz && j < ny;
       k = min(k, nz - 1);
@@ -239,8 +243,9 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   // T^n.  The batches are unrolled to keep those in registers.
   constexpr int NIT = FQ ? SEG / (4 * UQ) : 1;
   static_assert(!RY || (FQ && BUF), "RY: full-line class-map kernel");
-  static_assert(!PZ || (FQ && BUF && UQ == 2 && NT == 256 && SEG * G == 256 && LPB == 16),
-                "PZ: 16 x 16-cell lines, 16 lines per block");
+  static_assert(!PZ || (FQ && BUF && UQ == 2 && SEG == 16 &&
+                        ((G == 16 && LPB == 16) || (G == 32 && LPB == 4))),
+                "PZ: 16 x 16-cell lines in 16-line blocks or 16 x 32 in 4-line blocks");
   [[maybe_unused]] double hly[RY ? NIT : 1][UQ][4];
   [[maybe_unused]] uint32_t rxw[RY ? NIT : 1][UQ];
   // PZ with RY: a thread's four quads are rows j, j+1 of planes k, k+1 over
@@ -868,10 +873,11 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *T
       (std::is_same<V, ClassView>::value && SEG % 4 == 0 ? (size_t)LPB * SEG * G : 0);
   const bool full = x2_full(g, SEG * G, LPB, ty);
   if constexpr (std::is_same<V, ClassView>::value && SEG % 4 == 0) {
-    // 16 x 16-cell lines in 8 x 2 tiles: the paired line order (source-free
+    // 16-cell lanes, 16 x 16 in 8 x 2 tiles or 32 x 16 in 2 x 2 tiles: the
+    // paired line order (source-free
     // steps on full lines always take this R_y path: x_ry_ok holds exactly
     // when the lines are full)
-    constexpr bool kPZ = SEG == 16 && G == 16 && LPB == 16;
+    constexpr bool kPZ = SEG == 16 && ((G == 16 && LPB == 16) || (G == 32 && LPB == 4));
     if (ry) {  // the caller checked x_ry_ok and that the step has no sources
       if (!full || src) return hipErrorInvalidValue;
       if (kPZ && ty == 2)
@@ -898,7 +904,7 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *T
 
 XConfig x_config_for(int nx, int nz) {
   // two-phase kernel, 16 lanes per line where a line fits 16 x 16 cells;
-  // blocks of 8 or more lines as tiles 2 rows wide (when there are planes
+  // blocks of 4 or more lines as tiles 2 rows wide (when there are planes
   // enough to stack them)
   XConfig c{0, 0, 0, 0};
   if (nx <= 16) c = {1, 16, 8};
@@ -908,7 +914,7 @@ XConfig x_config_for(int nx, int nz) {
   else if (nx <= 256) c = {16, 16, 16};
   else if (nx <= 512) c = {16, 32, 4};
   else if (nx <= 1024) c = {16, 64, 2};
-  if (c.lpb >= 8 && 2 * nz >= c.lpb) c.ty = 2;
+  if (c.lpb >= 4 && 2 * nz >= c.lpb) c.ty = 2;
   return c;
 }
 

@@ -247,22 +247,25 @@ def test_y_from_x_path_matches_oracle(hs):
             assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, (x, n)
 
 
-def test_x_plane_pairs_match_oracle(hs):
-    """16 x 16-cell x lines in 8 x 2 tiles run with the lines ordered so that
-    each thread's quads are neighbours in y and z and share those rows
+@pytest.mark.parametrize("shape,x", [((16, 10, 256), (16, 16, 16)), ((4, 6, 512), (16, 32, 4))],
+                         ids=["256", "512"])
+def test_x_plane_pairs_match_oracle(hs, shape, x):
+    """x lines of 16 lanes x 16 cells in 8 x 2 tiles, or 32 lanes x 16 cells in
+    2 x 2 tiles, run with the lines ordered so that each thread's quads are
+    neighbours in y and z and share those rows
     (source-free steps 4-5; steps 0-3 carry sources and run the source
     kernel).  Against the oracle step by step, and bit for bit against
     1-row tiles, which take the same arithmetic per cell."""
     from heatsim2_amd import _lib, autotune
-    case = _random_case((16, 10, 256), seed=23)
+    case = _random_case(shape, seed=23)
     case["nsteps"] = 6
     args = to_package_args(hs, case)
     P, S = hs.setup(*args["setup"])
     ref = oracle.run_ref(case)
     rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
     out = {}
-    for x in ((16, 16, 16, 2), (16, 16, 16, 1)):
-        autotune._apply(_lib.load(), P.plan, {"x": x, "y": (16, 8), "z": (16, 8)})
+    for ty in (2, 1):
+        autotune._apply(_lib.load(), P.plan, {"x": x + (ty,), "y": (16, 8), "z": (4, 64)})
         assert _lib.plan_info(P.plan)["y_from_x"] == 1
         T = torch.from_numpy(case["T0"]).cuda()
         steps = []
@@ -270,9 +273,9 @@ def test_x_plane_pairs_match_oracle(hs):
             T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"],
                                  args["volumetric"])
             steps.append(T.cpu().numpy())
-            assert _rel(steps[-1], ref[n], rise) < TOL, (x, n)
-        out[x] = steps
-    for a, b in zip(out[(16, 16, 16, 2)], out[(16, 16, 16, 1)]):
+            assert _rel(steps[-1], ref[n], rise) < TOL, (ty, n)
+        out[ty] = steps
+    for a, b in zip(out[2], out[1]):
         assert np.array_equal(a, b)
 
 
