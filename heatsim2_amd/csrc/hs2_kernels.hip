@@ -140,6 +140,8 @@ __global__ __launch_bounds__(256) void k_sweep_x(Grid g, V view, const double *_
 // SEG consecutive cells per line, recomputes the x-axis coefficients from
 // the class bytes and solves like k_sweep_x.  LPB lines per block.  SRC adds
 // the volumetric source terms (only on the rare steps that carry one).
+// RY writes y's right-hand side instead of T*; PZ orders the block's lines
+// so that a thread's quads share their y and z neighbour rows in registers.
 template <class V, int SEG, int G, int LPB, bool SRC, bool FULL = false, bool BUF = false,
           bool RY = false, bool PZ = false>
 __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__restrict__ Tn,
