@@ -116,7 +116,10 @@ class adi_step(object):
 
     def _operators(self):
         if self._ops is None:
-            M, D, gm, gp = _equation_coefficients(self.ADI_params)
+            params = self.ADI_params
+            if getattr(params, "_eqcoef", None) is None:  # shared by the three substeps
+                params._eqcoef = _equation_coefficients(params)
+            M, D, gm, gp = params._eqcoef
             self._ops = reference_operators(self.ADI_params.shape, self.stepnum, M, D, gm, gp)
         return self._ops
 
