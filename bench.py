@@ -31,7 +31,9 @@ cpu_baseline: the oracle's reference-structured step (assembled permuted
 CSR operators, global Thomas with precomputed LU, transposes -- the
 upstream's structure) on one host thread, over a few steps of the same
 256^3 workload, reported as cell-steps/s, with the host CPUs available to
-the process beside the one it uses.
+the process beside the one it uses.  Also: the same one-thread step over the
+whole 40^3 steel-cube CPU workload (200 steps), and the oracle's OpenMP
+delta-form step on all host cores at 256^3 (4 steps).
 """
 import argparse
 import json
@@ -91,8 +93,24 @@ def cpu_baseline(case, nsteps):
         T = plan.step(T)
     dt = time.time() - t0
     del plan
+    # the same step over the whole CPU reference workload (40^3 steel cube,
+    # 200 steps), and the oracle's OpenMP delta-form step on all host cores
+    from hs2_cases import cases
+    c0 = cases.case_steel_cube(40, 200)
+    p0 = oracle.RefPlan(c0["shape"], oracle.assemble_case(c0))
+    T0 = np.array(c0["T0"])
+    t1 = time.time()
+    for _ in range(c0["nsteps"]):
+        T0 = p0.step(T0)
+    cube = int(np.prod(c0["shape"])) * c0["nsteps"] / (time.time() - t1)
+    del p0
+    t1 = time.time()
+    oracle.fast_run(coef, np.array(case["T0"]), 4)
+    omp = n * 4 / (time.time() - t1)
     return dict(value=n * nsteps / dt, unit="cell-steps/s", cores=1, kind="port",
                 host_cpus_available=len(os.sched_getaffinity(0)), host_cpus=os.cpu_count(),
+                steel_cube_40_200_steps=cube, omp_all_cores=omp,
+                omp_threads=int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
                 sample="%d source-free steps of the same %dx%dx%d two-layer workload through the "
                        "oracle's reference-structured step (permuted CSR B/C operators, global "
                        "Thomas with precomputed LU, transposes), one thread; %.1f s timed, "
