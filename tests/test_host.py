@@ -36,6 +36,26 @@ def test_library_exports_every_declared_function():
     assert L.hs2_version().decode().startswith("heatsim2_amd")
 
 
+@pytest.mark.parametrize("dims,spacing,code,msg", [
+    ((2, 2, 1025), 1e-3, 2, "nx=1025"),
+    ((2, 1025, 2), 1e-3, 2, "ny=1025"),
+    ((1025, 2, 2), 1e-3, 2, "nz=1025"),
+    ((1024, 1024, 2048), 1e-3, 2, "2^31"),
+    ((0, 4, 4), 1e-3, 1, "must be positive"),
+    ((4, 4, 4), 0.0, 1, "must be positive"),
+])
+def test_plan_create_rejects_before_touching_the_device(dims, spacing, code, msg):
+    """hs2_plan_create validates the grid (axis lengths up to 1024, fewer than
+    2^31 indices, positive sizes and spacings) before its first HIP call, so
+    these errors come back on a host without a GPU, with no plan handed out."""
+    L = _lib.load()
+    g = _lib.GridDesc(dims[0], dims[1], dims[2], spacing, 1e-3, 1e-3, 1e-3)
+    h = ctypes.c_void_p(123)
+    assert L.hs2_plan_create(ctypes.byref(g), ctypes.byref(h)) == code
+    assert h.value is None
+    assert msg in L.hs2_last_error().decode()
+
+
 def _compact_tables(case):
     pa = to_package_args(hs, case)
     a = pa["setup"]
