@@ -114,7 +114,11 @@ def cpu_baseline(case, nsteps):
                 sample="%d source-free steps of the same %dx%dx%d two-layer workload through the "
                        "oracle's reference-structured step (permuted CSR B/C operators, global "
                        "Thomas with precomputed LU, transposes), one thread; %.1f s timed, "
-                       "assembly %.1f s untimed" % ((nsteps,) + tuple(case["shape"]) + (dt, t_setup)))
+                       "assembly %.1f s untimed. steel_cube_40_200_steps: the same step over "
+                       "all 200 steps of the 40x40x40 steel cube, one thread. omp_all_cores: "
+                       "the oracle's OpenMP delta-form step, 4 steps of the %dx%dx%d workload "
+                       "on omp_threads threads" % ((nsteps,) + tuple(case["shape"]) + (dt, t_setup)
+                                                   + tuple(case["shape"])))
 
 
 # ---------------------------------------------------------------- batched
