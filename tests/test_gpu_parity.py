@@ -143,14 +143,16 @@ def _random_case(shape, seed):
 
 SHAPES = [(3, 5, 7), (2, 1, 1), (1, 4, 9), (5, 3, 64), (4, 6, 65), (3, 70, 130), (70, 3, 5),
           (6, 4, 256), (9, 17, 200), (130, 2, 33), (4, 300, 3), (260, 3, 2), (2, 2, 513),
-          (3, 512, 2), (517, 2, 3), (2, 1000, 2), (1024, 1, 1), (2, 3, 1024)]
+          (3, 512, 2), (517, 2, 3), (2, 1000, 2), (1024, 1, 1), (2, 3, 1024),
+          (2, 1024, 8), (1024, 2, 8)]
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 def test_random_problems_match_oracle(hs, shape):
     """Random materials (incl. FIXED), conducting / thin / insulating faces and
     sources on grids whose line lengths exercise every kernel variant
-    (1..1024 cells on every axis, non-multiples of 64)."""
+    (1..1024 cells on every axis, non-multiples of 64; 1024-cell y and z lines
+    whose 16-cell segments and 8-column blocks tile the grid exactly)."""
     case = _random_case(shape, seed=sum(shape))
     args = to_package_args(hs, case)
     P, S = hs.setup(*args["setup"])
