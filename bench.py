@@ -302,6 +302,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--timing-steps", type=int, default=500,
+                    help="steps per run of the per-kernel timing (hs2_run_timed)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, nargs=3, default=(256, 256, 256))
     ap.add_argument("--no-tune", action="store_true")
@@ -381,11 +383,17 @@ def main():
     log("timed %d steps: %.3f s, %.3f ms/step, %.3e cell-steps/s" % (K, wall, 1e3 * wall / K, value))
     assert torch.isfinite(T).all().item() and torch.isfinite(hist).all().item()
 
-    # per-kernel device time with HIP events on the launch stream
-    kms = (ctypes.c_double * 3)()
-    _lib.check("hs2_run_timed", L.hs2_run_timed(P.plan, _lib.ptr(T), K, kms, stream))
+    # per-kernel device time with HIP events on the launch stream, three ways
+    # (hs2_run_timed): the differential runs give `kernel_ms`
+    by_method = {}
+    for method, name in ((2, "differential"), (1, "event_pairs_no_fence"), (0, "event_pairs")):
+        arr = (ctypes.c_double * 4)()
+        _lib.check("hs2_run_timed", L.hs2_run_timed(P.plan, _lib.ptr(T), args.timing_steps,
+                                                    method, arr, stream))
+        by_method[name] = [round(v, 5) for v in arr]
     torch.cuda.synchronize()
-    kms = list(kms)
+    log("kernel ms (x, y, z, step) by timing method: %s" % by_method)
+    kms = by_method["differential"][:3]
     names = ("sweep_x", "sweep_y", "sweep_z")
     bpc = list(BYTES_PER_CELL)
     if tuned["y_from_x"]:
@@ -412,7 +420,8 @@ def main():
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs[dom] / HBM_PEAK_GBS, 4),
                     traffic=traffic, traffic_source=TRAFFIC_SUMMARY if traffic else None,
                     bytes_per_launch=bpc[dom] * n,
-                    kernel_ms=[round(x, 5) for x in kms],
+                    kernel_ms=[round(x, 5) for x in kms], kernel_ms_by_method=by_method,
+                    timing_steps=args.timing_steps,
                     step_achieved=round(step_gbs, 1), step_frac=round(step_gbs / HBM_PEAK_GBS, 4),
                     step_bytes_per_cell=sum(bpc),
                     step_frac_own_bytes=round(sum(bpc) * value / 1e9 / HBM_PEAK_GBS, 4),

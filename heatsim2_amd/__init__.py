@@ -27,8 +27,43 @@ import heatsim2_amd.autotune as autotune
 import heatsim2_amd.adjoint as adjoint
 import heatsim2_amd.batch as batch
 
-__version__ = "0.1.0"
+from heatsim2_amd._lib import LIBRARY_VERSION as __version__
 
 setup = crank_nicolson.setup
 run_adi_steps = alternatingdirection.run_adi_steps
 run_adi_steps_many = alternatingdirection.run_adi_steps_many
+
+
+# the upstream's module names, for plugins written against it
+_UPSTREAM_MODULES = ("expression", "boundary_conducting", "boundary_conducting_anisotropic",
+                     "boundary_insulating", "boundary_thininsulatinglayer", "surface_temperature",
+                     "hs2_indexing", "crank_nicolson", "alternatingdirection_c_pyx", "tridiag")
+
+
+def alias_as_heatsim2(install=True):
+    """Opt-in alias for boundary plugins written for the upstream package.
+
+    Such plugins do ``from heatsim2.expression import group``
+    (boundary_conducting.py:4 of the upstream).  alias_as_heatsim2()
+    registers this package under the name ``heatsim2`` (and its modules under
+    ``heatsim2.expression``, ``heatsim2.boundary_conducting``, ...), so that
+    those imports bind heatsim2_amd's expression objects; import the plugins
+    after calling it.  It refuses when a different ``heatsim2`` is already
+    imported.  alias_as_heatsim2(False) removes the alias again (modules
+    imported meanwhile keep the objects they bound).
+    """
+    import sys
+    me = sys.modules[__name__]
+    cur = sys.modules.get("heatsim2")
+    if not install:
+        if cur is me:
+            del sys.modules["heatsim2"]
+            for name in _UPSTREAM_MODULES:
+                sys.modules.pop("heatsim2." + name, None)
+        return
+    if cur is not None and cur is not me:
+        raise RuntimeError("another module named heatsim2 is already imported (%s); the alias "
+                           "would shadow it" % getattr(cur, "__file__", cur))
+    sys.modules["heatsim2"] = me
+    for name in _UPSTREAM_MODULES:
+        sys.modules["heatsim2." + name] = getattr(me, name)

@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhs2.so")
 
 HS2_OK = 0
+# the version string hs2_version() of a matching libhs2.so starts with
+LIBRARY_VERSION = "0.5"
 _ERRNAMES = {1: "HS2_E_ARG", 2: "HS2_E_UNSUPPORTED", 3: "HS2_E_HIP", 4: "HS2_E_STATE",
              5: "HS2_E_ALLOC"}
 
@@ -71,7 +73,7 @@ def load():
     L.hs2_step.restype = i32
     L.hs2_run.argtypes = [vp, vp, i32, vp, vp, i32, vp, vp]
     L.hs2_run.restype = i32
-    L.hs2_run_timed.argtypes = [vp, vp, i32, ctypes.POINTER(ctypes.c_double), vp]
+    L.hs2_run_timed.argtypes = [vp, vp, i32, i32, ctypes.POINTER(ctypes.c_double), vp]
     L.hs2_run_timed.restype = i32
     L.hs2_grad_work_doubles.argtypes = [vp, i32]
     L.hs2_grad_work_doubles.restype = ctypes.c_size_t
@@ -100,6 +102,10 @@ def load():
     L.hs2_last_error.restype = ctypes.c_char_p
     L.hs2_version.argtypes = []
     L.hs2_version.restype = ctypes.c_char_p
+    ver = L.hs2_version().decode()
+    if not ver.startswith("heatsim2_amd %s " % LIBRARY_VERSION):
+        raise ImportError("heatsim2_amd %s: %s reports '%s'; rebuild it" %
+                          (LIBRARY_VERSION, LIB_PATH, ver))
     _lib = L
     return L
 

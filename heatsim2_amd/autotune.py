@@ -88,9 +88,12 @@ def _store_cache(key, cfg):
 
 
 def _time(L, plan, T, steps):
-    ms = (ctypes.c_double * 3)()
-    _lib.check("hs2_run_timed", L.hs2_run_timed(plan, _lib.ptr(T), steps, ms, _lib.stream_handle()))
-    return list(ms)
+    """Per-launch x, y, z times from fence-free event pairs around every
+    kernel (hs2_run_timed method 1)."""
+    ms = (ctypes.c_double * 4)()
+    _lib.check("hs2_run_timed", L.hs2_run_timed(plan, _lib.ptr(T), steps, 1, ms,
+                                                _lib.stream_handle()))
+    return list(ms)[:3]
 
 
 def tune(ADI_params, T=None, steps=4, use_cache=True, verbose=False):

@@ -90,11 +90,15 @@ class GradBatch(object):
                                      device=_lib.device())
         return self._work
 
-    def surface_history(self, coefs, T0s, srcs=None):
-        """coefs: per-instance dicts (M, w, gz, gy, gx); T0s: initial fields;
-        srcs: per-instance source fields (W/m^3) or None.  Returns the list
-        of per-instance surface histories; gradients flow to the
-        coefficients and the source fields."""
+    def run(self, coefs, T0s, srcs=None):
+        """Forward only, no gradients: the final fields and the surface
+        histories of every instance (two lists of per-instance tensors)."""
+        packed = self._packed(coefs, T0s, srcs)
+        with torch.no_grad():
+            T, hist = _forward(self, *packed)
+        return self.unpack(T), self.unpack_hist(hist)
+
+    def _packed(self, coefs, T0s, srcs):
         M = self.pack([c["M"] for c in coefs])
         gz = self.pack([c["gz"] for c in coefs])
         gy = self.pack([c["gy"] for c in coefs])
@@ -102,7 +106,14 @@ class GradBatch(object):
         w = self.pack([c["w"] for c in coefs], dtype=torch.uint8)
         T0 = self.pack(T0s)
         src = self.pack(srcs) if srcs is not None else None
-        hist = _BatchHistory.apply(self, w, M, gz, gy, gx, T0, src)
+        return w, M, gz, gy, gx, T0, src
+
+    def surface_history(self, coefs, T0s, srcs=None):
+        """coefs: per-instance dicts (M, w, gz, gy, gx); T0s: initial fields;
+        srcs: per-instance source fields (W/m^3) or None.  Returns the list
+        of per-instance surface histories; gradients flow to the
+        coefficients and the source fields."""
+        hist = _BatchHistory.apply(self, *self._packed(coefs, T0s, srcs))
         return self.unpack_hist(hist)
 
     def __del__(self):
@@ -111,23 +122,31 @@ class GradBatch(object):
             self.handle = None
 
 
+def _forward(gb, w, M, gz, gy, gx, T0, src):
+    """hs2_batch_forward on packed fields: returns (final fields, histories)
+    and keeps the checkpoints in the batch's work buffer."""
+    L = _lib.load()
+    dev = _lib.device()
+    coef = tuple(t.detach().contiguous() for t in (M, w, gz, gy, gx))
+    _lib.check("hs2_batch_set_dense", L.hs2_batch_set_dense(gb.handle, *[_lib.ptr(t) for t in coef]))
+    gb._coef = coef
+    gb._generation += 1
+    T = T0.detach().contiguous().clone()
+    srcd = src.detach().contiguous() if src is not None else None
+    gb._src = srcd
+    hist = torch.empty(gb.H, dtype=torch.float64, device=dev)
+    work = gb.work()
+    _lib.check("hs2_batch_forward", L.hs2_batch_forward(
+        gb.handle, _lib.ptr(T), _lib.ptr(srcd), _lib.ptr(hist), _lib.ptr(work),
+        ctypes.c_size_t(gb.work_doubles), _lib.stream_handle()))
+    return T, hist
+
+
 class _BatchHistory(torch.autograd.Function):
     @staticmethod
     def forward(ctx, gb, w, M, gz, gy, gx, T0, src):
-        L = _lib.load()
-        dev = _lib.device()
-        coef = tuple(t.detach().contiguous() for t in (M, w, gz, gy, gx))
-        _lib.check("hs2_batch_set_dense", L.hs2_batch_set_dense(gb.handle, *[_lib.ptr(t) for t in coef]))
-        gb._coef = coef
-        gb._generation += 1
-        T = T0.detach().contiguous().clone()
-        srcd = src.detach().contiguous() if src is not None else None
-        hist = torch.empty(gb.H, dtype=torch.float64, device=dev)
-        work = gb.work()
-        _lib.check("hs2_batch_forward", L.hs2_batch_forward(
-            gb.handle, _lib.ptr(T), _lib.ptr(srcd), _lib.ptr(hist), _lib.ptr(work),
-            ctypes.c_size_t(gb.work_doubles), _lib.stream_handle()))
-        ctx.gb, ctx.coef, ctx.src, ctx.generation = gb, coef, srcd, gb._generation
+        _, hist = _forward(gb, w, M, gz, gy, gx, T0, src)
+        ctx.gb, ctx.coef, ctx.src, ctx.generation = gb, gb._coef, gb._src, gb._generation
         ctx.want_src = src is not None
         return hist
 

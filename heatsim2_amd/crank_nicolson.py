@@ -279,6 +279,20 @@ def _material_table(materials, dt):
     return fixed, kvals, M
 
 
+def _checked_law(q, b, axis):
+    """A face law's result must be a number or one of this package's
+    expressions; a plugin that built it from another package's expression
+    type (an upstream plugin importing the upstream's heatsim2.expression)
+    cannot be read."""
+    if isinstance(q, (numbers.Number, expression.linear_expression)):
+        return q
+    raise TypeError(
+        "boundary %d (%s faces): the face law returned a %s.%s, not a heatsim2_amd expression; "
+        "a plugin written for the upstream package imports heatsim2.expression -- call "
+        "heatsim2_amd.alias_as_heatsim2() before importing it"
+        % (b, _AXIS_NAME[axis], type(q).__module__, type(q).__name__))
+
+
 class _FaceLaws(object):
     """Face conductance per volume for (class, axis, k_minus, k_plus)."""
 
@@ -298,7 +312,7 @@ class _FaceLaws(object):
             KP = expression.linear_expression("kmat_p")
             Ts = [expression.linear_expression(n) for n in _T_ARGS[axis]]
             dz, dy, dx = self.d
-            self._expr[key] = fn(KM, KP, dz, dy, dx, *Ts, *extra)
+            self._expr[key] = _checked_law(fn(KM, KP, dz, dy, dx, *Ts, *extra), b, axis)
         return self._expr[key]
 
     def _alpha(self, expr, b, axis, km, kp):
@@ -345,7 +359,8 @@ class _FaceLaws(object):
         dz = self.d[0]
 
         def at(dy, dx):
-            return self._alpha(fn(KM, KP, dz, dy, dx, *Ts, *extra), b, axis, km, kp)
+            return self._alpha(_checked_law(fn(KM, KP, dz, dy, dx, *Ts, *extra), b, axis), b,
+                               axis, km, kp)
 
         a1, a2, a3 = at(1.0, 1.0), at(2.0, 1.0), at(1.0, 2.0)
         C = 2.0 * (a1 - a2)

@@ -17,10 +17,10 @@
 #include "hs2_launch.h"
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int code, const char *fmt, ...) {
+int hs2::fail(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -29,6 +29,9 @@ int fail(int code, const char *fmt, ...) {
   g_last_error = buf;
   return code;
 }
+
+namespace {
+using hs2::fail;
 
 #define HIP_TRY(expr)                                                                       \
   do {                                                                                      \
@@ -88,10 +91,12 @@ struct hs2_batch {
 
 namespace {
 
+}  // namespace
+
 // Untuned y / z shape: segments of up to 16 cells (32-cell segments spill
 // registers), and blocks of about 256 threads, as the tuner picks at 256 and
 // 512 cells per line.
-int default_strided(int L, hs2::SConfig *cfg) {
+int hs2::default_strided(int L, hs2::SConfig *cfg) {
   int s;
   if (L <= 4) s = 2;
   else if (L <= 16) s = 4;
@@ -105,6 +110,9 @@ int default_strided(int L, hs2::SConfig *cfg) {
   *cfg = {s, w};
   return 1;
 }
+
+namespace {
+using hs2::default_strided;
 
 bool strided_ok(int L, hs2::SConfig c) {
   const bool seg_ok = c.seg == 2 || c.seg == 4 || c.seg == 8 || c.seg == 16 || c.seg == 32;
@@ -434,7 +442,7 @@ extern "C" {
 
 const char *hs2_last_error(void) { return g_last_error.c_str(); }
 
-const char *hs2_version(void) { return "heatsim2_amd 0.4 (gfx950, fp64)"; }
+const char *hs2_version(void) { return "heatsim2_amd 0.5 (gfx950, fp64)"; }
 
 int hs2_plan_create(const hs2_grid_desc *grid, hs2_plan **out) {
   if (!grid || !out) return fail(HS2_E_ARG, "hs2_plan_create: null argument");
@@ -773,43 +781,77 @@ int hs2_run(hs2_plan *p, double *d_T, int nsteps, const uint8_t *d_vmap,
   return HS2_OK;
 }
 
-int hs2_run_timed(hs2_plan *p, double *d_T, int nsteps, double *kernel_ms, void *stream) {
-  if (!p || !d_T || !kernel_ms || nsteps < 1 || !aligned16(d_T))
+int hs2_run_timed(hs2_plan *p, double *d_T, int nsteps, int method, double *kernel_ms,
+                  void *stream) {
+  if (!p || !d_T || !kernel_ms || nsteps < 1 || !aligned16(d_T) || method < 0 || method > 2)
     return fail(HS2_E_ARG, "hs2_run_timed: bad argument");
   if (p->mode == MODE_NONE) return fail(HS2_E_STATE, "hs2_run_timed: coefficients not set");
   hipStream_t s = (hipStream_t)stream;
-  std::vector<hipEvent_t> ev((size_t)nsteps * 4, nullptr);
-  int rc = HS2_OK;
-  for (auto &e : ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
-      rc = fail(HS2_E_HIP, "hs2_run_timed: hipEventCreate");
-      break;
-    }
-  }
   const bool ry = use_ry(p, false);
-  for (int t = 0; t < nsteps && rc == HS2_OK; t++) {
-    hipEvent_t *e = &ev[(size_t)t * 4];
-    hipError_t err = hipEventRecord(e[0], s);
+  // the three substep kernels of one source-free step; `extra` launches kernel
+  // `twice` a second time first, into a scratch field (same inputs)
+  auto step = [&](int twice, double *extra) -> hipError_t {
+    hipError_t err = hipSuccess;
+    if (twice == 0) err = sweep_x(p, d_T, extra, nullptr, nullptr, nullptr, s, ry);
     if (err == hipSuccess) err = sweep_x(p, d_T, p->d_ws, nullptr, nullptr, nullptr, s, ry);
-    if (err == hipSuccess) err = hipEventRecord(e[1], s);
+    if (err == hipSuccess && twice == 1) err = sweep_s(p, 1, d_T, p->d_ws, extra, nullptr, s, ry);
     if (err == hipSuccess) err = sweep_s(p, 1, d_T, p->d_ws, p->d_ws, nullptr, s, ry);
-    if (err == hipSuccess) err = hipEventRecord(e[2], s);
+    if (err == hipSuccess && twice == 2) err = sweep_s(p, 0, d_T, p->d_ws, extra, nullptr, s);
     if (err == hipSuccess) err = sweep_s(p, 0, d_T, p->d_ws, d_T, nullptr, s);
-    if (err == hipSuccess) err = hipEventRecord(e[3], s);
-    if (err != hipSuccess) rc = fail(HS2_E_HIP, "hs2_run_timed: %s", hipGetErrorString(err));
-  }
-  if (rc == HS2_OK) {
-    hipError_t err = hipEventSynchronize(ev.back());
-    double acc[3] = {0, 0, 0};
+    return err;
+  };
+  const unsigned flags = method == 0 ? hipEventDefault : hipEventDisableSystemFence;
+  const int nev = method == 2 ? 8 : nsteps * 4;
+  std::vector<hipEvent_t> ev((size_t)nev, nullptr);
+  double *extra = nullptr;
+  int rc = HS2_OK;
+  for (auto &e : ev)
+    if (rc == HS2_OK && hipEventCreateWithFlags(&e, flags) != hipSuccess)
+      rc = fail(HS2_E_HIP, "hs2_run_timed: hipEventCreateWithFlags");
+  if (rc == HS2_OK && method == 2 && hipMalloc(&extra, sizeof(double) * (size_t)p->n) != hipSuccess)
+    rc = fail(HS2_E_ALLOC, "hs2_run_timed: scratch field");
+  hipError_t err = hipSuccess;
+  if (rc == HS2_OK && method < 2) {
+    // events around every substep kernel
+    for (int t = 0; t < nsteps && err == hipSuccess; t++) {
+      hipEvent_t *e = &ev[(size_t)t * 4];
+      err = hipEventRecord(e[0], s);
+      if (err == hipSuccess) err = sweep_x(p, d_T, p->d_ws, nullptr, nullptr, nullptr, s, ry);
+      if (err == hipSuccess) err = hipEventRecord(e[1], s);
+      if (err == hipSuccess) err = sweep_s(p, 1, d_T, p->d_ws, p->d_ws, nullptr, s, ry);
+      if (err == hipSuccess) err = hipEventRecord(e[2], s);
+      if (err == hipSuccess) err = sweep_s(p, 0, d_T, p->d_ws, d_T, nullptr, s);
+      if (err == hipSuccess) err = hipEventRecord(e[3], s);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    double acc[4] = {0, 0, 0, 0};
     for (int t = 0; t < nsteps && err == hipSuccess; t++)
       for (int k = 0; k < 3 && err == hipSuccess; k++) {
         float ms = 0.f;
         err = hipEventElapsedTime(&ms, ev[(size_t)t * 4 + k], ev[(size_t)t * 4 + k + 1]);
         acc[k] += ms;
       }
-    if (err != hipSuccess) rc = fail(HS2_E_HIP, "hs2_run_timed: %s", hipGetErrorString(err));
+    float all = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&all, ev.front(), ev.back());
     for (int k = 0; k < 3; k++) kernel_ms[k] = acc[k] / nsteps;
+    kernel_ms[3] = (double)all / nsteps;
+  } else if (rc == HS2_OK) {
+    // one event pair around each of four runs: plain, and x, y or z twice
+    float ms[4] = {0, 0, 0, 0};
+    for (int r = 0; r < 4 && err == hipSuccess; r++) {
+      err = hipEventRecord(ev[2 * r], s);
+      for (int t = 0; t < nsteps && err == hipSuccess; t++) err = step(r - 1, extra);
+      if (err == hipSuccess) err = hipEventRecord(ev[2 * r + 1], s);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    for (int r = 0; r < 4 && err == hipSuccess; r++)
+      err = hipEventElapsedTime(&ms[r], ev[2 * r], ev[2 * r + 1]);
+    for (int k = 0; k < 3; k++) kernel_ms[k] = (double)(ms[k + 1] - ms[0]) / nsteps;
+    kernel_ms[3] = (double)ms[0] / nsteps;
   }
+  if (rc == HS2_OK && err != hipSuccess)
+    rc = fail(HS2_E_HIP, "hs2_run_timed: %s", hipGetErrorString(err));
+  (void)hipFree(extra);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   return rc;

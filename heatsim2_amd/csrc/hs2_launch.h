@@ -15,6 +15,10 @@ struct SConfig {  // strided sweep: SEG cells per thread, CW lines per block
 };
 
 XConfig x_config_for(int nx, int nz);  // {0,0}: unsupported (nx > 1024)
+// untuned y / z shape for lines of L cells; 0 when L > 1024
+int default_strided(int L, SConfig *cfg);
+// record a message for hs2_last_error() on this thread; returns code
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 // ry: the x sweep writes y's right-hand side M T* - (Ly/2) T^n instead of T*
 // and the y sweep reads it (class maps, steps without sources, x shapes for
 // which x_ry_ok holds); the z sweep is unchanged

@@ -50,7 +50,7 @@ typedef struct {
  * hold a line in registers and LDS, and index with 32 bits. */
 int hs2_plan_create(const hs2_grid_desc *grid, hs2_plan **out);
 
-/* Release a plan.  Replaces delete_adi_step (alternatingdirection_c.h:49,
+/* Release a plan.  Replaces delete_adi_step (alternatingdirection_c.h:50,
  * alternatingdirection_c.c:14-54) and pyadi_step.__del__
  * (alternatingdirection_c_pyx.pyx:169-175). */
 void hs2_plan_destroy(hs2_plan *plan);
@@ -146,11 +146,23 @@ int hs2_step(hs2_plan *plan, const double *d_Tin, double *d_Tout, const uint8_t 
 int hs2_run(hs2_plan *plan, double *d_T, int nsteps, const uint8_t *d_vmap,
             const double *h_vtab_steps, int nvtab, double *d_surf_hist, void *stream);
 
-/* nsteps source-free steps in place on d_T, like hs2_run, with HIP events
- * recorded on `stream` around every substep kernel; kernel_ms[0..2] receive
- * the mean device time per launch of the x, y and z sweep.  A measurement
- * entry point with no upstream counterpart (the upstream has no timers). */
-int hs2_run_timed(hs2_plan *plan, double *d_T, int nsteps, double *kernel_ms, void *stream);
+/* Device time of the substep kernels, in milliseconds: nsteps source-free
+ * steps in place on d_T, like hs2_run, timed with HIP events on `stream`.
+ *   method 0: an event pair around every substep kernel (events with the
+ *             default system-scope fence, whose cache writeback and
+ *             invalidation lengthen the kernel each pair brackets);
+ *   method 1: the same with hipEventDisableSystemFence events;
+ *   method 2: four runs of nsteps steps, each between one event pair: plain,
+ *             and with the x, y or z kernel launched a second time per step
+ *             (into a scratch field the call allocates); a kernel's time is
+ *             its run's excess over the plain run per step.
+ * kernel_ms[0..2] receive the mean time per launch of the x, y and z sweep,
+ * kernel_ms[3] the mean time per step (plain run, or first to last event).
+ * The field advances 4*nsteps steps under method 2, nsteps otherwise.  A
+ * measurement entry point with no upstream counterpart (the upstream has no
+ * timers). */
+int hs2_run_timed(hs2_plan *plan, double *d_T, int nsteps, int method, double *kernel_ms,
+                  void *stream);
 
 /* ---- Gradients (no upstream counterpart: heatsim2 has no adjoint).
  * The differentiable time loop runs nsteps source-driven steps

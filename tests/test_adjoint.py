@@ -291,3 +291,16 @@ def test_per_voxel_forward_matches_class_forward(name):
     _, hist_ref = hs.run_adi_steps_many(P, 1e9, case["dt"], nsteps, T0, pa["vmap"],
                                         pa["volumetric"], surface_history=True)
     np.testing.assert_allclose(hist.cpu().numpy(), hist_ref.cpu().numpy(), rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("name", ["tiny_mixed", "tiny_aniso"])
+def test_oracle_rows_of_per_voxel_coefficients(name):
+    """hs2_cases.oracle_coef turns per-voxel coefficients into the oracle's
+    per-cell rows: equal to the oracle's own assembly of the same case,
+    COMPUTE and FIXED rows alike (the batched-path tests rely on it)."""
+    from hs2_cases import oracle_coef
+    case = load_golden(name)
+    kz, ky, kx, rhoc, fixed = _fields_of_case(case)
+    c = _coef_from_fields(case, kz, ky, kx, rhoc, fixed)
+    rows = oracle_coef(*(c[k].numpy() for k in ("M", "w", "gz", "gy", "gx")))
+    np.testing.assert_allclose(rows, oracle.assemble_case(case), rtol=1e-15, atol=0)

@@ -1,12 +1,14 @@
-"""Long runs against the oracle: every step count of the benchmark configs,
-at sizes the oracle (OpenMP delta-form step on the host cores) finishes in
-seconds; the full 256^3 x 1000-step and a larger laminate run when
-HS2_FULL_SIZE=1 (a few minutes of host time).
+"""Long runs against the oracle at the benchmark configurations' sizes:
+the 256^3 two-layer stack x 1000 steps (and 128^3), the 64x128x128 and
+32x64x64 laminates x 2000 steps, and the 128x512x512 laminate x 200 steps
+(all 2000 steps of it when HS2_FULL_SIZE=1: about 5 min of host time for
+the oracle).
 
 Bar: max |T_gpu - T_oracle| over the final field and over the surface
 history <= 1e-10 of the peak temperature rise (the history's maximum above
-the initial field).  The oracle's step is pinned to the upstream solver by
-the trajectory fixtures (tests/test_oracle.py).
+the initial field).  The oracle (OpenMP delta-form step on the host cores)
+is pinned to the upstream solver by the trajectory fixtures
+(tests/test_oracle.py).
 """
 import os
 
@@ -59,14 +61,23 @@ def _check(case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [128, pytest.param(256, marks=pytest.mark.skipif(
-    not FULL, reason="full size needs HS2_FULL_SIZE=1 (about 3 min of host time)"))])
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("n", [128, 256])
 def test_two_layer_1000_steps_matches_oracle(n):
     _check(cases.case_two_layer(nz=n, ny=n, nx=n, nsteps=1000))
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(32, 64, 64), pytest.param((64, 128, 128), marks=pytest.mark.skipif(
-    not FULL, reason="needs HS2_FULL_SIZE=1"))], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("shape", [(32, 64, 64), (64, 128, 128)],
+                         ids=lambda s: "x".join(map(str, s)))
 def test_laminate_2000_steps_matches_oracle(shape):
     _check(cases.case_laminate(*shape, nsteps=2000))
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("nsteps", [200, pytest.param(2000, marks=pytest.mark.skipif(
+    not FULL, reason="all 2000 steps need HS2_FULL_SIZE=1 (about 5 min of host time)"))])
+def test_laminate_128x512x512_matches_oracle(nsteps):
+    _check(cases.case_laminate(128, 512, 512, nsteps=nsteps))

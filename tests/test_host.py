@@ -33,7 +33,7 @@ def test_library_exports_every_declared_function():
     assert not missing, missing
     assert sorted(_lib.EXPORTS) == names
     L = _lib.load()
-    assert L.hs2_version().decode().startswith("heatsim2_amd")
+    assert L.hs2_version().decode().startswith("heatsim2_amd %s " % hs.__version__)
 
 
 @pytest.mark.parametrize("dims,spacing,code,msg", [
