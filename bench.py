@@ -17,10 +17,12 @@ field passes: x 16 B per cell, z 24 B; y 24 B, or 16 B when the x sweep
 hands it its right-hand side, as on this workload -- 56 B per cell-step in
 all, against the canonical 64 B of the delta-form step) over its mean
 launch duration, from HIP
-events recorded on the launch stream around every substep kernel of K more
-steps run right after the timed region (hs2_run_timed; each event pair adds a
-few microseconds of its own, so the achieved rate is slightly understated);
-peak = 8 TB/s.  traffic: the HBM bytes per launch of that kernel read from
+events recorded on the launch stream around every substep kernel of
+--timing-steps more steps run right after the timed region (hs2_run_timed,
+events created with hipEventDisableSystemFence: an event with the default
+system-scope fence writes back and invalidates the caches, which lengthens
+the kernel it brackets by a few microseconds; those timings are reported
+beside them); peak = 8 TB/s.  traffic: the HBM bytes per launch of that kernel read from
 the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
 (profiles/hbm_traffic_two_layer_256.txt, copy-calibrated).  copy_GBps: a
 device-to-device copy of one field on the same box (read + write bytes per
@@ -79,23 +81,32 @@ def device_copy_gbs(T, reps=20):
     return nbytes / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(case, nsteps):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+def _ref_step_rate(case, nsteps):
+    """cell-steps/s of the oracle's reference-structured step (one thread)
+    over nsteps source-free steps of a case, and the untimed assembly time"""
     import oracle
     t0 = time.time()
-    coef = oracle.assemble_case(case)
-    plan = oracle.RefPlan(case["shape"], coef)
+    plan = oracle.RefPlan(case["shape"], oracle.assemble_case(case))
     t_setup = time.time() - t0
     T = np.array(case["T0"])
-    n = int(np.prod(case["shape"]))
     t0 = time.time()
     for _ in range(nsteps):
         T = plan.step(T)
     dt = time.time() - t0
     del plan
+    return int(np.prod(case["shape"])) * nsteps / dt, dt, t_setup
+
+
+def cpu_baseline(case, nsteps, laminate_steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from hs2_cases import cases, load_golden
+    value, dt, t_setup = _ref_step_rate(case, nsteps)
+    n = int(np.prod(case["shape"]))
     # the same step over the whole CPU reference workload (40^3 steel cube,
-    # 200 steps), and the oracle's OpenMP delta-form step on all host cores
-    from hs2_cases import cases
+    # 200 steps), with the upstream's own wall time for it from the golden
+    # fixture: both were timed one after the other on the fixture-writing
+    # host, which gives the port's speed relative to the upstream solver
     c0 = cases.case_steel_cube(40, 200)
     p0 = oracle.RefPlan(c0["shape"], oracle.assemble_case(c0))
     T0 = np.array(c0["T0"])
@@ -104,21 +115,82 @@ def cpu_baseline(case, nsteps):
         T0 = p0.step(T0)
     cube = int(np.prod(c0["shape"])) * c0["nsteps"] / (time.time() - t1)
     del p0
+    fx = load_golden("steel_cube40")["fixture"]
+    port_over_ref = float(fx["ref_run_s"]) / float(fx["port_run_s"])
+    # the anisotropic laminate, 128 x 512 x 512, a few steps
+    lam = cases.case_laminate(128, 512, 512, nsteps=laminate_steps)
+    lam_rate, lam_dt, lam_setup = _ref_step_rate(lam, laminate_steps)
+    # the oracle's OpenMP delta-form step on all host cores
+    coef = oracle.assemble_case(case)
     t1 = time.time()
     oracle.fast_run(coef, np.array(case["T0"]), 4)
     omp = n * 4 / (time.time() - t1)
-    return dict(value=n * nsteps / dt, unit="cell-steps/s", cores=1, kind="port",
+    return dict(value=value, unit="cell-steps/s", cores=1, kind="port",
                 host_cpus_available=len(os.sched_getaffinity(0)), host_cpus=os.cpu_count(),
-                steel_cube_40_200_steps=cube, omp_all_cores=omp,
+                steel_cube_40_200_steps=cube,
+                port_speed_over_reference=round(port_over_ref, 4),
+                reference_equivalent_value=value / port_over_ref,
+                laminate_128x512x512=lam_rate,
+                omp_all_cores=omp,
                 omp_threads=int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
                 sample="%d source-free steps of the same %dx%dx%d two-layer workload through the "
                        "oracle's reference-structured step (permuted CSR B/C operators, global "
                        "Thomas with precomputed LU, transposes), one thread; %.1f s timed, "
                        "assembly %.1f s untimed. steel_cube_40_200_steps: the same step over "
-                       "all 200 steps of the 40x40x40 steel cube, one thread. omp_all_cores: "
-                       "the oracle's OpenMP delta-form step, 4 steps of the %dx%dx%d workload "
-                       "on omp_threads threads" % ((nsteps,) + tuple(case["shape"]) + (dt, t_setup)
-                                                   + tuple(case["shape"])))
+                       "all 200 steps of the 40x40x40 steel cube, one thread. "
+                       "port_speed_over_reference: the upstream solver's own time over the port's "
+                       "on that 40^3 x 200-step case, both timed on the fixture-writing host "
+                       "(%.3f s vs %.3f s; the upstream cannot run on this host); "
+                       "reference_equivalent_value = value / port_speed_over_reference. "
+                       "laminate_128x512x512: %d source-free steps of the 128x512x512 laminate "
+                       "through the same one-thread step (%.1f s timed, assembly %.1f s). "
+                       "omp_all_cores: the oracle's OpenMP delta-form step, 4 steps of the "
+                       "%dx%dx%d workload on omp_threads threads"
+                       % ((nsteps,) + tuple(case["shape"]) + (dt, t_setup) +
+                          (float(fx["ref_run_s"]), float(fx["port_run_s"]), laminate_steps,
+                           lam_dt, lam_setup) + tuple(case["shape"])))
+
+
+def strongly_coupled(hs, size, steps, log):
+    """The 256^3 step where the line solvers' early exit never fires: a steel
+    block (k=50, rho c=7850*470, 0.5 mm cells) at dt = 50 ms, k dt / (rho c
+    dx^2) = 2.7 on every axis, so every line's couplings after the first PCR
+    level stay far above 2^-64 and every level runs."""
+    import ctypes
+    import torch
+    from heatsim2_amd import _lib
+    from hs2_cases import cases, to_package_args
+    case = cases.case_steel_block(*size, nsteps=steps, dt=0.05)
+    pa = to_package_args(hs, case)
+    P, S = hs.setup(*pa["setup"])
+    dev = _lib.device()
+    T = torch.from_numpy(case["T0"]).to(dev)
+    cfg = hs.autotune.tune(P, T, use_cache=False)
+    T, _ = hs.run_adi_steps_many(P, 0.0, case["dt"], 5, T, pa["vmap"], pa["volumetric"])
+    L = _lib.load()
+    stream = _lib.stream_handle()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check("hs2_run", L.hs2_run(P.plan, _lib.ptr(T), steps, None, None, 1, None, stream))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    arr = (ctypes.c_double * 4)()
+    _lib.check("hs2_run_timed", L.hs2_run_timed(P.plan, _lib.ptr(T), steps, 1, arr, stream))
+    torch.cuda.synchronize()
+    assert torch.isfinite(T).all().item()
+    n = int(np.prod(size))
+    info = _lib.plan_info(P.plan)
+    P.release()
+    res = dict(workload="steel block %dx%dx%d, 0.5 mm cells, dt = 50 ms (k dt / (rho c dx^2) = "
+                        "2.7: every PCR level runs on every line)" % tuple(size),
+               value=n * steps / wall, unit="cell-steps/s", ms_per_step=1e3 * wall / steps,
+               steps=steps, kernel_ms=[round(v, 5) for v in arr][:3],
+               launch={k: info[k] for k in ("x_cpl", "x_lanes", "x_lines", "x_lines_y", "y_seg",
+                                            "y_lines", "z_seg", "z_lines", "y_from_x")},
+               step_frac=round(CANONICAL_STEP_BYTES * n * steps / wall / 1e9 / HBM_PEAK_GBS, 4))
+    log("strongly coupled: %.4f ms/step, %.3e cell-steps/s, kernels %s (tuned %s)"
+        % (res["ms_per_step"], res["value"], res["kernel_ms"], cfg))
+    return res
 
 
 # ---------------------------------------------------------------- batched
@@ -302,6 +374,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-laminate-steps", type=int, default=3)
     ap.add_argument("--timing-steps", type=int, default=500,
                     help="steps per run of the per-kernel timing (hs2_run_timed)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -311,11 +384,13 @@ def main():
                     help="force launch shapes, e.g. 'x=4,64,0;y=16,16;z=16,16' (skips the tuner)")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-gradient measurement")
+    ap.add_argument("--no-strong", action="store_true", help="skip the strongly coupled step")
+    ap.add_argument("--strong-steps", type=int, default=200)
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-seed", type=int, default=6)
     ap.add_argument("--ckpt-every", type=int, default=50)
-    ap.add_argument("--plain-sample", type=int, default=32,
+    ap.add_argument("--plain-sample", type=int, default=0,
                     help="time the plain path on this many instances and scale (0: all)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256^3 float64 field 20 times (scales HBM counters)")
@@ -383,17 +458,18 @@ def main():
     log("timed %d steps: %.3f s, %.3f ms/step, %.3e cell-steps/s" % (K, wall, 1e3 * wall / K, value))
     assert torch.isfinite(T).all().item() and torch.isfinite(hist).all().item()
 
-    # per-kernel device time with HIP events on the launch stream, three ways
-    # (hs2_run_timed): the differential runs give `kernel_ms`
+    # per-kernel device time from HIP events on the launch stream around every
+    # substep kernel (hs2_run_timed): fence-free events give `kernel_ms`;
+    # events with the default system-scope fence are reported beside them
     by_method = {}
-    for method, name in ((2, "differential"), (1, "event_pairs_no_fence"), (0, "event_pairs")):
+    for method, name in ((1, "event_pairs_no_fence"), (0, "event_pairs_default")):
         arr = (ctypes.c_double * 4)()
         _lib.check("hs2_run_timed", L.hs2_run_timed(P.plan, _lib.ptr(T), args.timing_steps,
                                                     method, arr, stream))
         by_method[name] = [round(v, 5) for v in arr]
     torch.cuda.synchronize()
     log("kernel ms (x, y, z, step) by timing method: %s" % by_method)
-    kms = by_method["differential"][:3]
+    kms = by_method["event_pairs_no_fence"][:3]
     names = ("sweep_x", "sweep_y", "sweep_z")
     bpc = list(BYTES_PER_CELL)
     if tuned["y_from_x"]:
@@ -441,6 +517,12 @@ def main():
     del T, hist
     P.release()
     torch.cuda.empty_cache()
+    out["strongly_coupled"] = None
+    if not args.no_strong:
+        log("strongly coupled step ...")
+        out["strongly_coupled"] = strongly_coupled(hs, (nz, ny, nx), args.strong_steps, log)
+        out["strongly_coupled"]["slowdown_vs_value"] = round(value / out["strongly_coupled"]["value"], 4)
+        torch.cuda.empty_cache()
     if not args.no_batch:
         log("batched gradients ...")
         out["batched"] = batched_gradients(hs, args.batch_instances, args.batch_seed,
@@ -450,7 +532,7 @@ def main():
         out["batched"] = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps, args.cpu_laminate_steps)
         log("cpu baseline %.3e cell-steps/s" % out["cpu_baseline"]["value"])
     else:
         out["cpu_baseline"] = None

@@ -191,3 +191,85 @@ def zgrid_for(case):
 
 def source_times(case):
     return [case["t0"] + n * case["dt"] for n in range(case["nsteps"])]
+
+
+def _grid_axis(lo, hi, n):
+    """Cell centres and spacing as the upstream's build_grid makes them
+    (__init__.py:58-65)."""
+    bnd, d = np.linspace(lo, hi, num=n + 1, retstep=True)
+    return bnd, bnd[:-1] + d / 2.0, float(d)
+
+
+def case_steelonfoam(nsteps=300):
+    """The upstream's steel-on-foam demo (demos/steelonfoam.py:10-112):
+    80x40x48 cells over 10.8 x 100 x 120 mm, 3.175 mm of steel (k=40,
+    rho=7750, c=466) on foam (k=0.4, rho=40, c=1500), an insulating gap of
+    10 x 10 mm in the steel/foam interface, insulating outer faces, a
+    1e4 J/m^2 flash into the top layer at t=0, T starting at 0, dt = 10 ms.
+    At this dt the steel's z lines are strongly coupled (k dt / (rho c dz^2)
+    about 6)."""
+    nz, ny, nx = 80, 40, 48
+    z_bnd, z, dz = _grid_axis(0.0, 10.8e-3, nz)
+    y_bnd, y, dy = _grid_axis(-0.05, 0.05, ny)
+    x_bnd, x, dx = _grid_axis(-0.06, 0.06, nx)
+    m, bz, by, bx, v = _zero(nz, ny, nx)
+    _outer(bz, by, bx, 1)
+    iface = int(np.argmin(np.abs(3.175e-3 - z_bnd)))
+    m[iface:] = 1
+    yg, xg = np.meshgrid(y, x, indexing="ij")
+    gap = (yg > 0.0) & (yg < 10e-3) & (xg > 0.0) & (xg < 10e-3)
+    bz[iface][gap] = 1
+    v[0] = 1
+    materials = [(0, 40.0, 40.0, 40.0, 7.75e3, 466.0, 0),
+                 (0, 0.4, 0.4, 0.4, 40.0, 1500.0, 0),
+                 (1, 0, 0, 0, 0, 0, 0)]
+    boundaries = [(BND_CONDUCTING, 0.0), (BND_INSULATING, 0.0)]
+    volumetric = [(NO_SOURCE,), (IMPULSE_SOURCE, 0.0, 10e3 / dz)]
+    return dict(name="steelonfoam", shape=(nz, ny, nx), d=(dz, dy, dx), dt=0.01, t0=0.0,
+                nsteps=nsteps, materials=materials, boundaries=boundaries,
+                volumetric=volumetric, maps=(m, bz, by, bx, v), T0=np.zeros((nz, ny, nx)),
+                matrices=False)
+
+
+def case_tutorial(nsteps=200):
+    """The upstream's tutorial (demos/testheatsim2.py:7-99): 5x50x55 cells
+    over 1.4 x 3.2 x 3.0 mm of titanium (k = diag(6.7, 6.7, 6.7) on the
+    anisotropic face law, rho=4430, c=526.3), an insulating delamination
+    at 0.4 mm depth over 2 x 1.2 mm, insulating outer faces, a 1e4 J/m^2
+    flash into the top layer at t=0, T starting at 0, dt = 10 ms.  At this
+    dt the x and y lines are strongly coupled (k dt / (rho c dx^2) about
+    10)."""
+    nz, ny, nx = 5, 50, 55
+    z_bnd, z, dz = _grid_axis(0.0, 1.4e-3, nz)
+    y_bnd, y, dy = _grid_axis(-1.6e-3, 1.6e-3, ny)
+    x_bnd, x, dx = _grid_axis(-1.5e-3, 1.5e-3, nx)
+    m, bz, by, bx, v = _zero(nz, ny, nx)
+    _outer(bz, by, bx, 1)
+    zz, yy, xx = np.meshgrid(z_bnd, y, x, indexing="ij")
+    delam = ((xx > -1e-3) & (xx < 1e-3) & (yy > -0.6e-3) & (yy < 0.6e-3) &
+             (zz == z_bnd[np.abs(z_bnd - 0.4e-3).argmin()]))
+    bz[delam] = 1
+    v[0] = 1
+    materials = [(0, 6.7, 6.7, 6.7, 4.43e3, 526.3, 1), (1, 0, 0, 0, 0, 0, 0)]
+    boundaries = [(BND_ANISO, 0.0), (BND_INSULATING, 0.0)]
+    volumetric = [(NO_SOURCE,), (IMPULSE_SOURCE, 0.0, 10e3 / dz)]
+    return dict(name="tutorial", shape=(nz, ny, nx), d=(dz, dy, dx), dt=0.010, t0=0.0,
+                nsteps=nsteps, materials=materials, boundaries=boundaries,
+                volumetric=volumetric, maps=(m, bz, by, bx, v), T0=np.zeros((nz, ny, nx)),
+                matrices=False)
+
+
+def case_steel_block(nz, ny, nx, nsteps, dt, dxyz=5e-4):
+    """A steel block (k=50, rho=7850, c=470) of 0.5 mm cells, insulating
+    faces, a 1e4 J/m^2 flash into the top layer at t=0, T0 = 293 K."""
+    d = (dxyz, dxyz, dxyz)
+    m, bz, by, bx, v = _zero(nz, ny, nx)
+    _outer(bz, by, bx, 1)
+    v[0] = 1
+    materials = [(0, 50.0, 50.0, 50.0, 7850.0, 470.0, 0)]
+    boundaries = [(BND_CONDUCTING, 0.0), (BND_INSULATING, 0.0)]
+    volumetric = [(NO_SOURCE,), (IMPULSE_SOURCE, 0.0, 1e4 / d[0])]
+    return dict(name="steel_%dx%dx%d_dt%g" % (nz, ny, nx, dt), shape=(nz, ny, nx), d=d, dt=dt,
+                t0=0.0, nsteps=nsteps, materials=materials, boundaries=boundaries,
+                volumetric=volumetric, maps=(m, bz, by, bx, v), T0=np.full((nz, ny, nx), 293.0),
+                matrices=False)

@@ -11,7 +11,9 @@ records, for the cases in cases.py:
     conventions);
   * end-to-end heatsim2.run_adi_steps trajectories
     (alternatingdirection_c_pyx.pyx:287-416), including every volumetric
-    source kind;
+    source kind and two of the upstream's own demos at their 10 ms step
+    (steel on foam, the tutorial), with the wall time of the upstream's run
+    and of the oracle's reference-structured step over the same steps;
   * central finite differences of a surface-history misfit computed with the
     upstream solver on a tiny grid where every voxel has its own material
     class, for the gradient checks.
@@ -145,11 +147,31 @@ def gen_trajectory_case(hs, case, zprofile_at=None, top_stride=4):
     out = dict(case_json=np.array(case_json(case)), T0=case["T0"], final=T,
                top_history=np.array(top), top_stride=np.array(top_stride),
                surf_mean=np.array(surf_mean), ref_setup_s=np.array(t_setup),
-               ref_run_s=np.array(t_run), **maps_dict(case))
+               ref_run_s=np.array(t_run), port_run_s=np.array(port_run_seconds(case)),
+               **maps_dict(case))
     if zprofile_at is not None:
         out["zprofile"] = np.array(zprof)
         out["zprofile_at"] = np.array(zprofile_at)
     return out
+
+
+def port_run_seconds(case):
+    """Wall time of the oracle's reference-structured step (one thread) over
+    the same steps, sources evaluated per step as the upstream does, right
+    after the upstream's own run on the same host: the port's speed relative
+    to the upstream's."""
+    sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+    import oracle
+    plan = oracle.RefPlan(case["shape"], oracle.assemble_case(case))
+    vol = oracle.case_volume(case)
+    zg = oracle.zgrid(case["shape"], case["d"][0])
+    T = np.array(case["T0"], dtype=np.float64)
+    t0 = time.time()
+    for n in range(case["nsteps"]):
+        t = case["t0"] + n * case["dt"]
+        S = oracle.source_field(case["volumetric"], case["maps"][4], t, case["dt"], vol, zg)
+        T = plan.step(T, S)
+    return time.time() - t0
 
 
 # ---------------------------------------------------------------- gradients
@@ -260,7 +282,7 @@ def gen_grad_case(hs, nvox=10, rel=3e-3):
 def main(argv):
     hs = _refload.load_reference()
     todo = argv or ["tiny_mixed", "tiny_aniso", "tiny_curved", "steel_cube40",
-                    "two_layer_mini", "laminate_mini", "grad_tiny"]
+                    "two_layer_mini", "laminate_mini", "steelonfoam", "tutorial", "grad_tiny"]
     for name in todo:
         t = time.time()
         if name == "tiny_mixed":
@@ -278,6 +300,11 @@ def main(argv):
         elif name == "laminate_mini":
             c = cases.case_laminate(nz=12, ny=24, nx=20, nsteps=80, patch_min=3, patch_max=10)
             out = gen_trajectory_case(hs, c, zprofile_at=(10, 9), top_stride=1)
+        elif name == "steelonfoam":
+            out = gen_trajectory_case(hs, cases.case_steelonfoam(), zprofile_at=(2, 2))
+        elif name == "tutorial":
+            out = gen_trajectory_case(hs, cases.case_tutorial(), zprofile_at=(20, 21),
+                                      top_stride=2)
         elif name == "grad_tiny":
             out = gen_grad_case(hs)
         else:

@@ -9,9 +9,15 @@ is its surface-temperature history plus N(0, 0.01^2) K noise (seed 4); the
 loss is the L2 misfit at the homogeneous guess kz = 0.6.  Gradients with
 respect to per-voxel kz, ky, kx, rho*c and the flash amplitude E are checked
 against Richardson-extrapolated central differences of the device forward
-at 32 random voxels (seed 5) of the top four layers -- the surface history
-senses a voxel through z diffusion, whose length over the 1 s window is
-about 1.6 cells, so deeper voxels carry gradients at round-off level.
+at 32 random voxels (seed 5) of the top four layers per parameter.  The
+surface history senses a voxel through z diffusion, whose length over the
+1 s window is about 1.6 cells, so deeper voxels carry gradients at
+round-off level.  The lateral conductivities matter only where the field
+varies laterally (delamination edges, the kz texture): their voxels are
+drawn among the top-four-layer voxels whose gradient is within 1e-3 of
+the largest, where a central difference resolves it.  The loss difference of a
+central difference is formed as sum((h+ - h-) (h+ + h- - 2 obs)), which
+keeps its round-off at the scale of the history change instead of the loss.
 Bar: |g - g_fd| <= 1e-6 max(|g_fd|, 1e-3 max|g_fd|).
 """
 import numpy as np
@@ -66,11 +72,14 @@ def _loss(env, p):
     return ((_history(env, p) - env["obs"]) ** 2).sum()
 
 
-def _richardson(f, x0, rel=3e-3):
+def _richardson(hist_at, obs, x0, rel=3e-3):
+    """(4 D(h) - D(2h)) / 3 of the loss, D(h) = (L(x0+h) - L(x0-h)) / 2h with
+    L(a) - L(b) = sum((h_a - h_b) (h_a + h_b - 2 obs))."""
+    def D(h):
+        a, b = hist_at(x0 + h), hist_at(x0 - h)
+        return float(((a - b) * (a + b - 2.0 * obs)).sum()) / (2 * h)
     h = rel * abs(x0)
-    d1 = (f(x0 + h) - f(x0 - h)) / (2 * h)
-    d2 = (f(x0 + 2 * h) - f(x0 - 2 * h)) / (4 * h)
-    return (4 * d1 - d2) / 3
+    return (4 * D(h) - D(2 * h)) / 3
 
 
 def _grad_ok(g, fd):
@@ -88,28 +97,35 @@ def test_laminate_gradients_match_central_differences():
     loss = _loss(env, p)
     loss.backward()
     nz, ny, nx = env["shape"]
-    vox = np.random.default_rng(5).choice(4 * ny * nx, size=32, replace=False)
-    for name in ("kz", "ky", "kx", "rhoc"):
-        g = p[name].grad.detach().cpu().numpy().ravel()[vox]
+    top = 4 * ny * nx
+    rng = np.random.default_rng(5)
+    for name in ("kz", "rhoc", "ky", "kx"):
+        gall = p[name].grad.detach().cpu().numpy().ravel()
+        if name in ("kz", "rhoc"):
+            vox = rng.choice(top, size=32, replace=False)
+        else:
+            live = np.flatnonzero(np.abs(gall[:top]) >= 1e-3 * np.abs(gall[:top]).max())
+            vox = rng.choice(live, size=min(32, live.size), replace=False)
+        g = gall[vox]
         fd = []
         for v in vox:
-            def f(val, v=v):
+            def hist_at(val, v=v):
                 q = {k: t.detach().clone() for k, t in p.items()}
                 q[name].view(-1)[int(v)] = val
                 with torch.no_grad():
-                    return float(_loss(env, q))
-            fd.append(_richardson(f, float(base[name].view(-1)[int(v)])))
+                    return _history(env, q)
+            fd.append(_richardson(hist_at, env["obs"], float(base[name].view(-1)[int(v)])))
         ok, worst = _grad_ok(g, fd)
-        print("d/d%s at 32 voxels: max |g| %.3e, worst %.3f of the tolerance"
-              % (name, np.abs(g).max(), worst), flush=True)
+        print("d/d%s at %d voxels: |g| %.3e .. %.3e, worst %.3f of the tolerance"
+              % (name, len(vox), np.abs(g).min(), np.abs(g).max(), worst), flush=True)
         assert ok, (name, vox, g, fd, worst)
 
-    def fE(val):
+    def hist_E(val):
         q = {k: t.detach().clone() for k, t in p.items()}
         q["E"] = torch.tensor(val, dtype=torch.float64, device=dev)
         with torch.no_grad():
-            return float(_loss(env, q))
-    fdE = _richardson(fE, float(base["E"]))
+            return _history(env, q)
+    fdE = _richardson(hist_E, env["obs"], float(base["E"]))
     ok, worst = _grad_ok([float(p["E"].grad)], [fdE])
     print("d/dE: adjoint %.9e, fd %.9e" % (float(p["E"].grad), fdE), flush=True)
     assert ok, (float(p["E"].grad), fdE, worst)
