@@ -38,7 +38,7 @@ EXPORTS = ("hs2_plan_create", "hs2_plan_destroy", "hs2_plan_set_compact", "hs2_p
            "hs2_run_timed", "hs2_grad_work_doubles", "hs2_grad_forward", "hs2_grad_backward",
            "hs2_batch_create", "hs2_batch_destroy", "hs2_batch_layout", "hs2_batch_set_dense",
            "hs2_batch_set_options", "hs2_batch_forward", "hs2_batch_backward",
-           "hs2_surface_temperature", "hs2_last_error", "hs2_version")
+           "hs2_step_noalloc", "hs2_surface_temperature", "hs2_last_error", "hs2_version")
 
 _lib = None
 
@@ -96,6 +96,8 @@ def load():
     L.hs2_batch_forward.restype = i32
     L.hs2_batch_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]
     L.hs2_batch_backward.restype = i32
+    L.hs2_step_noalloc.argtypes = [ctypes.POINTER(GridDesc)] + [vp] * 10
+    L.hs2_step_noalloc.restype = i32
     L.hs2_surface_temperature.argtypes = [vp, i32, i32, i32, dbl, vp, vp]
     L.hs2_surface_temperature.restype = i32
     L.hs2_last_error.argtypes = []

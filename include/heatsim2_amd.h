@@ -252,6 +252,23 @@ int hs2_batch_backward(hs2_batch *batch, const double *d_src, const double *d_gr
                        double *d_gM, double *d_gz, double *d_gy, double *d_gx, double *d_gsrc,
                        double *d_work, size_t work_doubles, void *stream);
 
+/* One Douglas step on per-voxel coefficients that allocates nothing: no
+ * plan, no hipMalloc or hipFree, no allocating library routine.  It has no
+ * counterpart in the upstream; it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * Same numbers and kernels as hs2_step on a plan given these coefficients
+ * with hs2_plan_set_dense (d_M, d_w, d_gz, d_gy, d_gx as there), with a
+ * source field d_sfield (n doubles, W m^-3, or NULL) and the readout d_surf
+ * ((ny,nx), or NULL).  d_Tin may equal d_Tout.  The workspace is a fixed
+ * device array of 2^21 cells (16 MiB) that each GPU the library runs on
+ * holds until the process ends: grids above 2^21 cells are refused with
+ * HS2_E_UNSUPPORTED, and calls that overlap on one GPU share the array and
+ * race (serialise them, e.g. on one stream).  grid->dt is not used. */
+int hs2_step_noalloc(const hs2_grid_desc *grid, const double *d_M, const uint8_t *d_w,
+                     const double *d_gz, const double *d_gy, const double *d_gx,
+                     const double *d_Tin, double *d_Tout, const double *d_sfield, double *d_surf,
+                     void *stream);
+
 /* Insulating-top surface readout of a (nz,ny,nx) field into (ny,nx):
  * T0 - 5/16 (T1 - T0).  Replaces insulating_z_min_surface_temperature
  * (surface_temperature.py:4-36). */
