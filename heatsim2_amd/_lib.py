@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhs2.so")
 
 HS2_OK = 0
+HS2_OPT_COMBINED_X = 1
 # the version string hs2_version() of a matching libhs2.so starts with
 LIBRARY_VERSION = "0.5"
 _ERRNAMES = {1: "HS2_E_ARG", 2: "HS2_E_UNSUPPORTED", 3: "HS2_E_HIP", 4: "HS2_E_STATE",
@@ -33,7 +34,8 @@ class HS2Error(RuntimeError):
 
 # every symbol include/heatsim2_amd.h declares
 EXPORTS = ("hs2_plan_create", "hs2_plan_destroy", "hs2_plan_set_compact", "hs2_plan_set_dense",
-           "hs2_plan_set_dense_asym", "hs2_plan_expand", "hs2_plan_set_config", "hs2_plan_info",
+           "hs2_plan_set_dense_asym", "hs2_plan_expand", "hs2_plan_set_config", "hs2_plan_set_option",
+           "hs2_plan_info",
            "hs2_step", "hs2_run",
            "hs2_run_timed", "hs2_grad_work_doubles", "hs2_grad_forward", "hs2_grad_backward",
            "hs2_batch_create", "hs2_batch_destroy", "hs2_batch_layout", "hs2_batch_set_dense",
@@ -67,6 +69,8 @@ def load():
     L.hs2_plan_expand.restype = i32
     L.hs2_plan_set_config.argtypes = [vp, i32, i32, i32, i32, i32, i32, i32, i32]
     L.hs2_plan_set_config.restype = i32
+    L.hs2_plan_set_option.argtypes = [vp, i32, i32]
+    L.hs2_plan_set_option.restype = i32
     L.hs2_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.hs2_plan_info.restype = i32
     L.hs2_step.argtypes = [vp, vp, vp, vp, vp, i32, vp, vp, vp]
@@ -152,8 +156,8 @@ def surface_temperature_device(T, dz):
 
 
 def plan_info(plan):
-    info = (ctypes.c_int * 13)()
+    info = (ctypes.c_int * 14)()
     check("hs2_plan_info", load().hs2_plan_info(plan, info))
     keys = ("mode", "classes_z", "classes_y", "classes_x", "x_cpl", "x_lanes", "y_seg",
-            "y_lines", "z_seg", "z_lines", "x_lines", "x_lines_y", "y_from_x")
+            "y_lines", "z_seg", "z_lines", "x_lines", "x_lines_y", "y_from_x", "x_combined")
     return dict(zip(keys, list(info)))

@@ -52,6 +52,11 @@ struct hs2_plan {
   int device = 0;
   int mode = MODE_NONE;
   hs2::ClassView cv{};
+  hs2::CombView comb{};         // x sweep's combined class plane (class mode)
+  bool have_comb = false;       // built (the class triples number <= 256)
+  bool use_comb = true;         // hs2_plan_set_option(HS2_OPT_COMBINED_X)
+  uint8_t *d_comb = nullptr;    // n combined ids
+  double *d_ctab = nullptr;     // 12 doubles per id
   hs2::DenseView dv{};
   hs2::AsymView av{};
   int ncls[3] = {0, 0, 0};
@@ -192,6 +197,9 @@ hs2::Grid grid_of(const hs2_plan *p) { return hs2::Grid{p->grid.nz, p->grid.ny, 
 hipError_t sweep_x(const hs2_plan *p, const double *Tn, double *Ts, const uint8_t *vmap,
                    const double *vtab, const double *sfield, hipStream_t s, bool ry = false) {
   const hs2::Grid g = grid_of(p);
+  if (p->mode == MODE_CLASS && p->have_comb && p->use_comb && !vmap && !sfield &&
+      hs2::x_ry_ok(p->xcfg, g))
+    return hs2::launch_sweep_x(p->xcfg, g, p->comb, Tn, Ts, s, ry);
   if (p->mode == MODE_CLASS)
     return hs2::launch_sweep_x(p->xcfg, g, p->cv, Tn, Ts, vmap, vtab, sfield, s, ry);
   if (p->mode == MODE_ASYM)
@@ -273,6 +281,11 @@ void free_coefficients(hs2_plan *p) {
   p->d_lut = nullptr;
   p->d_dense = nullptr;
   p->d_densew = nullptr;
+  (void)hipFree(p->d_comb);
+  (void)hipFree(p->d_ctab);
+  p->d_comb = nullptr;
+  p->d_ctab = nullptr;
+  p->have_comb = false;
   p->mode = MODE_NONE;
 }
 
@@ -436,6 +449,67 @@ int batch_replay(hs2_batch *b, hipGraphExec_t *exec, std::vector<const void *> *
   return HS2_OK;
 }
 
+// The x sweep's combined class plane: one id per cell for its (z, y, x)
+// class triple, when the triples number 256 or fewer (else the x sweep
+// reads the three class planes).  tabs/off: the per-axis class tables.
+int build_combined(hs2_plan *p, const std::vector<double> &tabs, const int off[4]) {
+  constexpr size_t kWords = (size_t(1) << 24) / 32;
+  const int n = p->n;
+  const uint8_t *cz = p->cv.cls[0], *cy = p->cv.cls[1], *cx = p->cv.cls[2];
+  uint32_t *d_bits = nullptr;
+  int *d_rank = nullptr;
+  std::vector<uint32_t> bits(kWords);
+  std::vector<int> rank0(kWords);
+  std::vector<uint32_t> keys;
+  auto done = [&](int rc) {
+    (void)hipFree(d_bits);
+    (void)hipFree(d_rank);
+    return rc;
+  };
+  hipError_t e = hipMalloc(&d_bits, kWords * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(d_bits, 0, kWords * sizeof(uint32_t));
+  if (e == hipSuccess) e = hs2::launch_comb_mark(n, cz, cy, cx, d_bits, 0);
+  if (e == hipSuccess)
+    e = hipMemcpy(bits.data(), d_bits, kWords * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return done(fail(HS2_E_HIP, "combined class plane: %s", hipGetErrorString(e)));
+  int cnt = 0;
+  for (size_t w = 0; w < kWords; w++) {
+    rank0[w] = cnt;
+    for (uint32_t b = bits[w]; b; b &= b - 1, cnt++)
+      keys.push_back((uint32_t)(w * 32 + __builtin_ctz(b)));
+  }
+  if (cnt > hs2::kMaxClass) return done(HS2_OK);  // the three planes serve
+  std::vector<double> ctab((size_t)hs2::CombView::kStride * cnt, 0.0);
+  for (int i = 0; i < cnt; i++) {
+    const uint32_t id[3] = {keys[i] & 0xffu, (keys[i] >> 8) & 0xffu, keys[i] >> 16};  // z, y, x
+    for (int a = 0; a < 3; a++) {
+      const double *src = &tabs[off[a] + (size_t)id[a] * hs2::kTabStride];
+      double *dst = &ctab[(size_t)i * hs2::CombView::kStride + 4 * (2 - a)];
+      for (int k = 0; k < 3; k++) dst[k] = src[k];
+    }
+  }
+  e = hipMalloc(&d_rank, kWords * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(d_rank, rank0.data(), kWords * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&p->d_comb, (size_t)n);
+  if (e == hipSuccess) e = hipMalloc(&p->d_ctab, sizeof(double) * ctab.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(p->d_ctab, ctab.data(), sizeof(double) * ctab.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hs2::launch_comb_assign(n, cz, cy, cx, d_bits, d_rank, p->d_comb, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(p->d_comb);
+    (void)hipFree(p->d_ctab);
+    p->d_comb = nullptr;
+    p->d_ctab = nullptr;
+    return done(fail(HS2_E_HIP, "combined class plane: %s", hipGetErrorString(e)));
+  }
+  p->comb.comb = p->d_comb;
+  p->comb.tab = p->d_ctab;
+  p->comb.n = cnt;
+  p->have_comb = true;
+  return done(HS2_OK);
+}
+
 }  // namespace
 
 extern "C" {
@@ -582,7 +656,7 @@ int hs2_plan_set_compact(hs2_plan *p, const uint8_t *d_mat, const uint8_t *d_bz,
     p->cv.tab = p->d_tab;
     for (int t = 0; t < 4; t++) p->cv.off[t] = off[t];
     p->mode = MODE_CLASS;
-    return HS2_OK;
+    return build_combined(p, tabs, off);
   }
   // more than 256 classes on an axis: per-cell float64 coefficients
   double *d_gtab = nullptr, *d_mtab = nullptr;
@@ -673,6 +747,8 @@ int hs2_plan_info(const hs2_plan *p, int *info) {
   info[9] = p->scfg[0].cw;
   info[11] = p->xcfg.ty;
   info[12] = use_ry(p, false) ? 1 : 0;
+  info[13] = p->mode == MODE_CLASS && p->have_comb && p->use_comb &&
+             hs2::x_ry_ok(p->xcfg, grid_of(p)) ? p->comb.n : 0;
   return HS2_OK;
 }
 
@@ -694,6 +770,17 @@ int hs2_plan_set_config(hs2_plan *p, int x_cpl, int x_lanes, int x_lines, int x_
   p->scfg[1] = yc;
   p->scfg[0] = zc;
   return HS2_OK;
+}
+
+int hs2_plan_set_option(hs2_plan *p, int option, int value) {
+  if (!p) return fail(HS2_E_ARG, "hs2_plan_set_option: null plan");
+  switch (option) {
+    case HS2_OPT_COMBINED_X:
+      p->use_comb = value != 0;
+      return HS2_OK;
+    default:
+      return fail(HS2_E_ARG, "hs2_plan_set_option: unknown option %d", option);
+  }
 }
 
 int hs2_plan_expand(hs2_plan *p, double *d_M, double *d_w, double *d_gz, double *d_gy,

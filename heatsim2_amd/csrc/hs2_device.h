@@ -74,6 +74,39 @@ struct ClassView {
   }
 };
 
+// One uint8 id per cell for all three axes: the id of the cell's (z, y, x)
+// class triple, indexing a <=256-entry table whose entry holds the three
+// axis entries (M, w, g, pad) side by side, x at 0, y at 4, z at 8.  For the
+// x sweep, which needs every axis of a cell: one class plane instead of
+// three (built by the plan when the triples number 256 or fewer).
+struct CombView {
+  static constexpr bool kAsym = false;
+  static constexpr int kStride = 12;  // doubles per id
+  const uint8_t *__restrict__ comb;
+  const double *__restrict__ tab;
+  int n;  // ids
+  using Raw = uint32_t;
+  __host__ __device__ int lds_doubles(int) const { return kStride * n; }
+  __device__ void load_tables(double *lds, int) const {
+    for (int t = threadIdx.x; t < kStride * n; t += blockDim.x) lds[t] = tab[t];
+  }
+  __device__ __forceinline__ const double *table(const double *lds, int axis) const {
+    return lds + 4 * (2 - axis);
+  }
+  __device__ __forceinline__ Raw fetch(int, int c) const { return comb[c]; }
+  __device__ __forceinline__ void fetch4(int, int c, Raw (&r)[4]) const {
+    const uint32_t w = *reinterpret_cast<const uint32_t *>(comb + c);
+    r[0] = w & 0xffu;
+    r[1] = (w >> 8) & 0xffu;
+    r[2] = (w >> 16) & 0xffu;
+    r[3] = w >> 24;
+  }
+  __device__ __forceinline__ Entry resolve(const double *t, Raw r) const {
+    const double2 mw = *reinterpret_cast<const double2 *>(t + r * kStride);
+    return Entry{mw.x, mw.y, t[r * kStride + 2]};
+  }
+};
+
 struct DenseRaw {
   double M, g;
   uint32_t w;

@@ -21,6 +21,8 @@
 //
 // Every global load goes through a clamped in-range address with no branch
 // around it, so a thread issues all of its loads before it first waits.
+#include <algorithm>
+
 #include "hs2_device.h"
 #include "hs2_batch.h"
 #include "hs2_launch.h"
@@ -180,8 +182,12 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
     const unsigned ncell = (unsigned)nlines * (unsigned)nx;
     bT = make_buf(Tn, ncell * 8u);
     bS = make_buf(Ts, ncell * 8u);
+    if constexpr (std::is_same<V, CombView>::value) {
+      bC[0] = bC[1] = bC[2] = make_buf(view.comb, ncell);
+    } else {
 #pragma unroll
-    for (int a = 0; a < 3; a++) bC[a] = make_buf(view.cls[a], ncell);
+      for (int a = 0; a < 3; a++) bC[a] = make_buf(view.cls[a], ncell);
+    }
   }
   if (tid < LPB) {
     // ty = 0: the block's lines follow each other in (k, j) order.  ty > 0:
@@ -256,6 +262,11 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   constexpr bool PZ4 = PZ && RY && NIT == 2;
   [[maybe_unused]] double cT1[2][4], cT2[2][4];
   [[maybe_unused]] typename V::Raw cRz2[2][4];
+  // COMB: one id per cell for all three axes, so a row's ids are one word
+  // (own, y+1, z+1 rows: three loads per quad instead of five), and the
+  // second batch's own word is the first batch's z+1 word
+  constexpr bool COMB = std::is_same<V, CombView>::value;
+  [[maybe_unused]] uint32_t cWz[2];
   if ((nx & 3) == 0) {
     const int qpl = FQ ? SEG * G / 4 : nx >> 2;
     const int total = LPB * qpl;
@@ -309,7 +320,9 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
             const double2 v1 = buf_load_d2(bT, o[k] * 8u + 16u);
             (*dst[k])[0] = v0.x; (*dst[k])[1] = v0.y; (*dst[k])[2] = v1.x; (*dst[k])[3] = v1.y;
           }
-          const uint32_t wx = buf_load_u32(bC[2], o[0]);
+          uint32_t wx;
+          if constexpr (COMB && PZ4) wx = it == 1 ? cWz[b] : buf_load_u32(bC[2], o[0]);
+          else wx = buf_load_u32(bC[2], o[0]);
           unpack4(wx, rx[b]);
           if constexpr (RY) rxw[it][b] = wx;
           cbuf[li[b] * (SEG * G / 4) + xq[b]] = wx;
@@ -329,10 +342,22 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
             TR[b] = buf_load_d(bT, (unsigned)(b0 + xr) * 8u);
             rxn[b] = buf_load_u8(bC[2], (unsigned)(b0 + xr));
           }
-          unpack4(buf_load_u32(bC[1], o[0]), ry[b]);
-          if (!(PZ && b == 0)) unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
-          if (!(PZ4 && it == 1)) unpack4(buf_load_u32(bC[0], o[0]), rz[b]);
-          unpack4(buf_load_u32(bC[0], o[4]), rzp[b]);
+          if constexpr (COMB) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              ry[b][q] = rx[b][q];
+              rz[b][q] = rx[b][q];
+            }
+            if (!(PZ && b == 0)) unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
+            const uint32_t wzp = buf_load_u32(bC[0], o[4]);
+            unpack4(wzp, rzp[b]);
+            if constexpr (PZ4) cWz[b] = wzp;
+          } else {
+            unpack4(buf_load_u32(bC[1], o[0]), ry[b]);
+            if (!(PZ && b == 0)) unpack4(buf_load_u32(bC[1], o[2]), ryp[b]);
+            if (!(PZ4 && it == 1)) unpack4(buf_load_u32(bC[0], o[0]), rz[b]);
+            unpack4(buf_load_u32(bC[0], o[4]), rzp[b]);
+          }
         } else {
         const double2 *p0 = reinterpret_cast<const double2 *>(Tn + b0 + x0);
         const double2 *p1 = reinterpret_cast<const double2 *>(Tn + lbase[lb][1] + x0);
@@ -643,7 +668,8 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
                                                     const uint8_t *__restrict__ vmap,
                                                     const double *__restrict__ vtab_g,
                                                     const double *__restrict__ sfield, int ty) {
-  constexpr bool BUF = FULL && SEG % 4 == 0 && std::is_same<V, ClassView>::value;
+  constexpr bool BUF = FULL && SEG % 4 == 0 &&
+                      (std::is_same<V, ClassView>::value || std::is_same<V, CombView>::value);
   sweep_x2_body<V, SEG, G, LPB, SRC, FULL, BUF, RY, PZ>(g, view, Tn, Ts, vmap, vtab_g, sfield,
                                                         xcd_remap(blockIdx.x, gridDim.x), ty);
 }
@@ -942,6 +968,45 @@ static hipError_t launch_x_v(XConfig cfg, const Grid &g, const V &v, const doubl
   return hipErrorInvalidValue;
 }
 
+// x sweep on the combined class plane: the full-line buffer kernels only
+// (source-free steps; x_ry_ok(cfg) holds), with or without R_y
+template <int SEG, int G, int LPB>
+static hipError_t launch_x2c_t(const Grid &g, const CombView &v, int ty, const double *Tn,
+                               double *Ts, bool ry, hipStream_t s) {
+  const int tz = ty > 0 ? LPB / ty : 0;
+  const int blocks = ty > 0 ? ((g.nz + tz - 1) / tz) * ((g.ny + ty - 1) / ty)
+                            : (g.nz * g.ny + LPB - 1) / LPB;
+  constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;
+  const size_t shm = sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + LPB * RS) +
+                     (size_t)LPB * SEG * G;
+  if (!x2_full(g, SEG * G, LPB, ty)) return hipErrorInvalidValue;
+  constexpr bool kPZ = SEG == 16 && ((G == 16 && LPB == 16) || (G == 32 && LPB == 4));
+  const uint8_t *vmap = nullptr;
+  const double *vtab = nullptr, *sfield = nullptr;
+  if (ry && kPZ && ty == 2)
+    hipLaunchKernelGGL((k_sweep_x2<CombView, SEG, G, LPB, false, true, true, kPZ>), dim3(blocks),
+                       dim3(G * LPB), shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
+  else if (ry)
+    hipLaunchKernelGGL((k_sweep_x2<CombView, SEG, G, LPB, false, true, true>), dim3(blocks),
+                       dim3(G * LPB), shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
+  else
+    hipLaunchKernelGGL((k_sweep_x2<CombView, SEG, G, LPB, false, true>), dim3(blocks),
+                       dim3(G * LPB), shm, s, g, v, Tn, Ts, vmap, vtab, sfield, ty);
+  return hipGetLastError();
+}
+
+hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const CombView &v, const double *Tn,
+                          double *Ts, hipStream_t s, bool ry) {
+#define HS2_XC(C, GG, LL)                                                          \
+  if (cfg.lpb == LL && cfg.cpl == C && cfg.g == GG)                                \
+    return launch_x2c_t<C, GG, LL>(g, v, cfg.ty, Tn, Ts, ry, s);
+  HS2_XC(4, 16, 8) HS2_XC(8, 16, 8) HS2_XC(16, 16, 8) HS2_XC(4, 16, 16) HS2_XC(8, 16, 16)
+  HS2_XC(16, 16, 4) HS2_XC(16, 16, 16) HS2_XC(8, 32, 4) HS2_XC(16, 32, 4) HS2_XC(16, 32, 2)
+  HS2_XC(16, 64, 2) HS2_XC(8, 64, 2)
+#undef HS2_XC
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const ClassView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
                           const double *sfield, hipStream_t s, bool ry) {
@@ -1040,6 +1105,44 @@ hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const AsymView &v, i
                                 const double *Tn, const double *Y, double *X, double *surf,
                                 double dz, hipStream_t s, bool ry) {
   return launch_s_v(cfg, g, v, axis, Tn, Y, X, surf, dz, ry, s);
+}
+
+// triple key of a cell: z class | y class << 8 | x class << 16
+__global__ void k_comb_mark(int n, const uint8_t *__restrict__ cz, const uint8_t *__restrict__ cy,
+                            const uint8_t *__restrict__ cx, uint32_t *bits) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const uint32_t key = cz[c] | ((uint32_t)cy[c] << 8) | ((uint32_t)cx[c] << 16);
+    atomicOr(bits + (key >> 5), 1u << (key & 31));
+  }
+}
+
+// comb[c] = rank of the cell's key among the marked keys (rank0[w]: marked
+// keys below word w); the caller checked that there are 256 or fewer
+__global__ void k_comb_assign(int n, const uint8_t *__restrict__ cz,
+                              const uint8_t *__restrict__ cy, const uint8_t *__restrict__ cx,
+                              const uint32_t *__restrict__ bits, const int *__restrict__ rank0,
+                              uint8_t *comb) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const uint32_t key = cz[c] | ((uint32_t)cy[c] << 8) | ((uint32_t)cx[c] << 16);
+    const uint32_t below = bits[key >> 5] & ((1u << (key & 31)) - 1u);
+    comb[c] = (uint8_t)(rank0[key >> 5] + __popc(below));
+  }
+}
+
+hipError_t launch_comb_mark(int n, const uint8_t *cz, const uint8_t *cy, const uint8_t *cx,
+                            uint32_t *bits, hipStream_t s) {
+  const int blocks = std::min((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_comb_mark, dim3(blocks), dim3(256), 0, s, n, cz, cy, cx, bits);
+  return hipGetLastError();
+}
+
+hipError_t launch_comb_assign(int n, const uint8_t *cz, const uint8_t *cy, const uint8_t *cx,
+                              const uint32_t *bits, const int *rank0, uint8_t *comb,
+                              hipStream_t s) {
+  const int blocks = std::min((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_comb_assign, dim3(blocks), dim3(256), 0, s, n, cz, cy, cx, bits, rank0,
+                     comb);
+  return hipGetLastError();
 }
 
 hipError_t launch_classify(const Grid &g, const uint8_t *mat, const uint8_t *bz, const uint8_t *by,

@@ -27,6 +27,9 @@ bool x_ry_ok(XConfig c, const Grid &g);
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const ClassView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
                           const double *sfield, hipStream_t s, bool ry = false);
+// the combined class plane (full-line source-free steps, x_ry_ok(cfg, g))
+hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const CombView &v, const double *Tn,
+                          double *Ts, hipStream_t s, bool ry);
 hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const DenseView &v, const double *Tn,
                           double *Ts, const uint8_t *vmap, const double *vtab,
                           const double *sfield, hipStream_t s, bool ry = false);
@@ -47,6 +50,13 @@ hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const AsymView &v, i
                                 const double *Tn, const double *Y, double *X, double *surf,
                                 double dz, hipStream_t s, bool ry = false);
 
+// K0 of the combined plane: mark the (z, y, x) class triples that occur in a
+// 2^24-bit map, then give each cell the rank of its triple among them
+hipError_t launch_comb_mark(int n, const uint8_t *cz, const uint8_t *cy, const uint8_t *cx,
+                            uint32_t *bits, hipStream_t s);
+hipError_t launch_comb_assign(int n, const uint8_t *cz, const uint8_t *cy, const uint8_t *cx,
+                              const uint32_t *bits, const int *rank0, uint8_t *comb,
+                              hipStream_t s);
 hipError_t launch_classify(const Grid &g, const uint8_t *mat, const uint8_t *bz, const uint8_t *by,
                            const uint8_t *bx, const uint8_t *lut, const uint8_t *edge, int nmat,
                            int lutlen, uint8_t *cz, uint8_t *cy, uint8_t *cx, hipStream_t s);

@@ -117,11 +117,22 @@ int hs2_plan_expand(hs2_plan *plan, double *d_M, double *d_w, double *d_gz, doub
 int hs2_plan_set_config(hs2_plan *plan, int x_cpl, int x_lanes, int x_lines, int x_lines_y,
                         int y_seg, int y_lines, int z_seg, int z_lines);
 
-/* info[0..12] = mode (0 none, 1 class maps, 2 dense, 3 dense per side),
+/* Plan options (no upstream counterpart).
+ *   HS2_OPT_COMBINED_X (default 1): with class maps, let the x sweep of
+ *     source-free steps read one combined class byte per cell (the id of
+ *     the cell's z, y, x class triple, built by hs2_plan_set_compact when
+ *     there are 256 triples or fewer) instead of the three class planes;
+ *     0 reads the three planes.  Same numbers either way. */
+enum { HS2_OPT_COMBINED_X = 1 };
+int hs2_plan_set_option(hs2_plan *plan, int option, int value);
+
+/* info[0..13] = mode (0 none, 1 class maps, 2 dense, 3 dense per side),
  * classes per axis z/y/x, x_cpl, x_lanes, y_seg, y_lines, z_seg, z_lines,
- * x_lines, x_lines_y, and 1 when steps without sources hand the y sweep its
+ * x_lines, x_lines_y, 1 when steps without sources hand the y sweep its
  * right-hand side M T* - (Ly/2) T^n from the x sweep (16 B per cell less field
- * traffic per step than writing T* and re-reading T^n), else 0. */
+ * traffic per step than writing T* and re-reading T^n) else 0, and the
+ * number of combined class ids the x sweep of source-free steps reads (0:
+ * it reads the three class planes). */
 int hs2_plan_info(const hs2_plan *plan, int *info);
 
 /* One full Douglas step T^n -> T^{n+1} (x, y, z substeps).

@@ -281,6 +281,54 @@ def test_x_plane_pairs_match_oracle(hs, shape, x):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("maker,x", [
+    (lambda: cases.case_laminate(16, 10, 256, nsteps=6, patch_min=2, patch_max=8), (16, 16, 16, 2)),
+    (lambda: cases.case_laminate(16, 10, 256, nsteps=6, patch_min=2, patch_max=8), (16, 16, 16, 0)),
+    (lambda: cases.case_two_layer(nz=4, ny=6, nx=512, nsteps=6, steel_layers=2), (16, 32, 4, 2)),
+    (lambda: cases.case_two_layer(nz=6, ny=8, nx=128, nsteps=6, steel_layers=3), (8, 16, 8, 0))],
+    ids=["256-pairs", "256-rows", "512-pairs", "128"])
+def test_x_combined_class_plane_matches_three_planes(hs, maker, x):
+    """Source-free steps on full x lines read one combined class byte per
+    cell (its z, y, x class triple) instead of three planes: bit for bit the
+    same steps as the three planes (HS2_OPT_COMBINED_X off), and the oracle's."""
+    from heatsim2_amd import _lib, autotune
+    case = maker()
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    ref = oracle.run_ref(case)
+    rise = max(np.abs(np.array(ref) - case["T0"]).max(), 1e-300)
+    L = _lib.load()
+    out = {}
+    for comb in (1, 0):
+        _lib.check("hs2_plan_set_option", L.hs2_plan_set_option(P.plan, _lib.HS2_OPT_COMBINED_X,
+                                                                comb))
+        autotune._apply(L, P.plan, {"x": x, "y": (8, 16), "z": (4, 32)})
+        info = _lib.plan_info(P.plan)
+        assert (info["x_combined"] > 0) == bool(comb), info
+        T = torch.from_numpy(case["T0"]).cuda()
+        steps = []
+        for n in range(case["nsteps"]):
+            T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"],
+                                 args["volumetric"])
+            steps.append(T.cpu().numpy())
+            assert _rel(steps[-1], ref[n], rise) < TOL, (comb, n)
+        out[comb] = steps
+    for a, b in zip(out[1], out[0]):
+        assert np.array_equal(a, b)
+
+
+def test_x_combined_plane_needs_few_triples(hs):
+    """Random maps give more than 256 class triples: no combined plane, the
+    x sweep reads the three planes (and still matches the oracle in the
+    random-problem tests)."""
+    from heatsim2_amd import _lib
+    case = _random_case((8, 16, 128), seed=5)
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    info = _lib.plan_info(P.plan)
+    assert info["mode"] == 1 and info["x_combined"] == 0
+
+
 @pytest.mark.parametrize("name", ["tiny_mixed", "tiny_aniso", "tiny_curved"])
 def test_step_operators_match_upstream(hs, name):
     """ADI_steps[s].Amat/Bmat/Cmats/Dvec of a device plan (per-cell

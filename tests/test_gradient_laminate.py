@@ -22,7 +22,7 @@ The lateral conductivities matter only where the field varies laterally
 a central difference of this 1e7 loss resolves them to about 5e-9 (its
 loss difference carries ~1e-9 of round-off from the two forward runs).
 Their 32 voxels are drawn among the top-four-layer voxels whose gradient is
-at least a tenth of the largest, with steps of 1.5e-2 of the value, and the
+at least a quarter of the largest, with steps of 2e-2 of the value, and the
 bar is 1e-5 max(|g_fd|, 1e-3 max|g_fd|).  (kx and ky meet the 1e-6 bar on
 the smaller grids of tests/test_adjoint.py, against the upstream solver's
 and the device's differences.)
@@ -112,9 +112,9 @@ def test_laminate_gradients_match_central_differences():
             vox = rng.choice(top, size=32, replace=False)
             rel, bar = 3e-3, 1e-6
         else:
-            live = np.flatnonzero(np.abs(gall[:top]) >= 0.1 * np.abs(gall[:top]).max())
+            live = np.flatnonzero(np.abs(gall[:top]) >= 0.25 * np.abs(gall[:top]).max())
             vox = rng.choice(live, size=min(32, live.size), replace=False)
-            rel, bar = 1.5e-2, 1e-5
+            rel, bar = 2e-2, 1e-5
         g = gall[vox]
         fd = []
         for v in vox:
