@@ -22,7 +22,9 @@ events recorded on the launch stream around every substep kernel of
 events created with hipEventDisableSystemFence: an event with the default
 system-scope fence writes back and invalidates the caches, which lengthens
 the kernel it brackets by a few microseconds; those timings are reported
-beside them); peak = 8 TB/s.  traffic: the HBM bytes per launch of that kernel read from
+beside them), less the dispatch gap each fence-free pair still adds (the
+pairs' sum over the step time of a run with no events between kernels,
+split in three: event_pair_overhead_ms); peak = 8 TB/s.  traffic: the HBM bytes per launch of that kernel read from
 the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
 (profiles/hbm_traffic_two_layer_256.txt, copy-calibrated).  copy_GBps: a
 device-to-device copy of one field on the same box (read + write bytes per
@@ -52,7 +54,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = (16.0, 24.0, 24.0)  # x, y, z sweep algorithmic bytes (y: 16 from x's R_y)
 CANONICAL_STEP_BYTES = 64.0  # the delta-form step's field passes, the metric's yardstick
-TRAFFIC_SUMMARY = "hbm_traffic_two_layer_256.txt"
+# committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries per workload
+TRAFFIC_SUMMARY = {"two_layer": "hbm_traffic_two_layer_256.txt",
+                   "laminate": "hbm_traffic_laminate_128x512x512.txt"}
 METRIC = "FP64 ADI cell-steps/s @256³ + batched fwd+adjoint gradients/s vs plain"
 
 
@@ -378,7 +382,10 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=500,
                     help="steps per run of the per-kernel timing (hs2_run_timed)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--size", type=int, nargs=3, default=(256, 256, 256))
+    ap.add_argument("--workload", choices=("two_layer", "laminate"), default="two_layer",
+                    help="two_layer: the 256^3 steel/foam stack (the metric's); laminate: the "
+                         "128x512x512 anisotropic laminate")
+    ap.add_argument("--size", type=int, nargs=3, default=None)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--config", default=None,
                     help="force launch shapes, e.g. 'x=4,64,0;y=16,16;z=16,16' (skips the tuner)")
@@ -420,8 +427,19 @@ def main():
         print(json.dumps(res), flush=True)
         return
 
+    if args.size is None:
+        args.size = (256, 256, 256) if args.workload == "two_layer" else (128, 512, 512)
     nz, ny, nx = args.size
-    case = cases.case_two_layer(nz=nz, ny=ny, nx=nx, nsteps=1000)
+    if args.workload == "two_layer":
+        case = cases.case_two_layer(nz=nz, ny=ny, nx=nx, nsteps=1000)
+        wl = ("two-layer steel/foam stack %dx%dx%d (z,y,x), dx=0.5 mm, thin layer h=250 W/m^2K, "
+              "flash %.1f J/m^2, dt=0.5 ms, FP64, surface readout every step"
+              % (nz, ny, nx, case["flash_J_m2"]))
+    else:
+        case = cases.case_laminate(nz, ny, nx, nsteps=2000)
+        wl = ("anisotropic laminate %dx%dx%d (z,y,x), kz=0.6, kx=ky=5 W/mK, rho c=1550*1200, "
+              "%d thin-layer delaminations (h=100 W/m^2K), flash 1e4 J/m^2, dt=2 ms, FP64, "
+              "surface readout every step" % (nz, ny, nx, case["npatch"]))
     pa = to_package_args(hs, case)
     t0 = time.time()
     P, S = hs.setup(*pa["setup"])
@@ -462,14 +480,23 @@ def main():
     # substep kernel (hs2_run_timed): fence-free events give `kernel_ms`;
     # events with the default system-scope fence are reported beside them
     by_method = {}
-    for method, name in ((1, "event_pairs_no_fence"), (0, "event_pairs_default")):
+    for method, name in ((1, "event_pairs_no_fence"), (0, "event_pairs_default"),
+                         (2, "differential")):
         arr = (ctypes.c_double * 4)()
         _lib.check("hs2_run_timed", L.hs2_run_timed(P.plan, _lib.ptr(T), args.timing_steps,
                                                     method, arr, stream))
         by_method[name] = [round(v, 5) for v in arr]
     torch.cuda.synchronize()
     log("kernel ms (x, y, z, step) by timing method: %s" % by_method)
-    kms = by_method["event_pairs_no_fence"][:3]
+    # each fence-free event pair still adds its own dispatch gap to the kernel
+    # it brackets: the three pairs' sum exceeds the step time of a run with no
+    # events between its kernels (the differential method's plain run, one
+    # event pair around all its steps) by that overhead, taken off in equal
+    # parts
+    ev = by_method["event_pairs_no_fence"]
+    plain_step = by_method["differential"][3]
+    pair_overhead = max(0.0, (ev[0] + ev[1] + ev[2] - plain_step) / 3.0)
+    kms = [v - pair_overhead for v in ev[:3]]
     names = ("sweep_x", "sweep_y", "sweep_z")
     bpc = list(BYTES_PER_CELL)
     if tuned["y_from_x"]:
@@ -488,15 +515,17 @@ def main():
     # measured HBM bytes per launch: committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # summary of this workload (copy-calibrated), when it is for this size
     from heatsim2_amd.rocprof_summary import read_traffic
-    tr = read_traffic(os.path.join(ROOT, "profiles", TRAFFIC_SUMMARY))
+    tr = read_traffic(os.path.join(ROOT, "profiles", TRAFFIC_SUMMARY[args.workload]))
     traffic = None
     if tr and tr.get("cells") == n:
         traffic = tr["traffic_bytes"].get(names[dom])
     roofline = dict(bound="hbm", kernel=names[dom], achieved=round(gbs[dom], 1),
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs[dom] / HBM_PEAK_GBS, 4),
-                    traffic=traffic, traffic_source=TRAFFIC_SUMMARY if traffic else None,
+                    traffic=traffic,
+                    traffic_source=TRAFFIC_SUMMARY[args.workload] if traffic else None,
                     bytes_per_launch=bpc[dom] * n,
                     kernel_ms=[round(x, 5) for x in kms], kernel_ms_by_method=by_method,
+                    event_pair_overhead_ms=round(pair_overhead, 5),
                     timing_steps=args.timing_steps,
                     step_achieved=round(step_gbs, 1), step_frac=round(step_gbs / HBM_PEAK_GBS, 4),
                     step_bytes_per_cell=sum(bpc),
@@ -506,13 +535,12 @@ def main():
     out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": K,
            "warmup": args.warmup, "ms_per_step": 1e3 * wall / K, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": "two-layer steel/foam stack %dx%dx%d (z,y,x), dx=0.5 mm, thin "
-                                  "layer h=250 W/m^2K, flash %.1f J/m^2, dt=0.5 ms, FP64, surface "
-                                  "readout every step" % (nz, ny, nx, case["flash_J_m2"]),
+           "config": {"workload": wl,
                       "cells": n, "parallelism": "single GPU",
                       "launch": {k: tuned[k] for k in ("x_cpl", "x_lanes", "x_lines",
                                                        "x_lines_y", "y_seg", "y_lines", "z_seg",
-                                                       "z_lines", "mode", "y_from_x")}},
+                                                       "z_lines", "mode", "y_from_x",
+                                                       "x_combined")}},
            "roofline": roofline}
     del T, hist
     P.release()

@@ -1110,9 +1110,16 @@ hipError_t launch_sweep_strided(SConfig cfg, const Grid &g, const AsymView &v, i
 // triple key of a cell: z class | y class << 8 | x class << 16
 __global__ void k_comb_mark(int n, const uint8_t *__restrict__ cz, const uint8_t *__restrict__ cy,
                             const uint8_t *__restrict__ cx, uint32_t *bits) {
+  // few distinct keys over many cells: a wave whose lanes share one key sets
+  // it once, and a bit already seen set is not set again (a stale read only
+  // costs an extra atomic), so the atomics do not pile up on a few words
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
     const uint32_t key = cz[c] | ((uint32_t)cy[c] << 8) | ((uint32_t)cx[c] << 16);
-    atomicOr(bits + (key >> 5), 1u << (key & 31));
+    const uint32_t first = __shfl(key, __builtin_ctzll(__ballot(1)), kWave);
+    const bool one = __all(key == first);
+    const bool lead = (threadIdx.x & (kWave - 1)) == (unsigned)__builtin_ctzll(__ballot(1));
+    if ((!one || lead) && !(bits[key >> 5] & (1u << (key & 31))))
+      atomicOr(bits + (key >> 5), 1u << (key & 31));
   }
 }
 

@@ -8,6 +8,12 @@
         HBM bytes per sweep launch from FETCH_SIZE / WRITE_SIZE (KiB per
         dispatch), each scaled by what the same counter reads for a plain
         device copy of B bytes
+    python -m heatsim2_amd.rocprof_summary kernels --trace DIR [--fetch DIR
+            --write DIR --copy-fetch DIR --copy-write DIR --copy-bytes B]
+            [--match SUBSTRING]
+        every kernel (or those whose name holds SUBSTRING): launches, total
+        and mean device time, and with the counter runs the copy-scaled HBM
+        bytes per launch and in total
 
 Kernels are attributed to sweeps by name (the strided kernel's third
 template argument is its axis).  The last line of the traffic
@@ -165,6 +171,48 @@ def traffic_summary(a):
     return "\n".join(lines)
 
 
+def _per_kernel(dispatches):
+    by = collections.defaultdict(list)
+    for _, name, v in dispatches:
+        by[short_name(name)].append(v)
+    return by
+
+
+def kernels_summary(a):
+    trace = _rows(_one(a.trace, "_kernel_trace.csv"))
+    dur = _per_kernel([(int(r["Dispatch_Id"]), r["Kernel_Name"],
+                        (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+                       for r in trace])
+    have_pmc = bool(a.fetch and a.write and a.copy_fetch and a.copy_write and a.copy_bytes)
+    fetch = write = {}
+    ff = fw = 1.0
+    lines = []
+    if have_pmc:
+        _, cf, _ = _copy_reading(a.copy_fetch, "FETCH_SIZE")
+        _, cw, _ = _copy_reading(a.copy_write, "WRITE_SIZE")
+        ff = a.copy_bytes / (cf * 1024.0)
+        fw = a.copy_bytes / (cw * 1024.0)
+        fetch = _per_kernel(_counter(a.fetch, "FETCH_SIZE"))
+        write = _per_kernel(_counter(a.write, "WRITE_SIZE"))
+        lines.append("counter scale factors (device copy of %d B): fetch x%.3f, write x%.3f"
+                     % (a.copy_bytes, ff, fw))
+    hdr = "%-60s %7s %10s %10s" % ("kernel", "calls", "total_ms", "mean_us")
+    if have_pmc:
+        hdr += " %11s %11s %10s" % ("read_MB", "write_MB", "total_GB")
+    lines.append(hdr)
+    rows = sorted(dur.items(), key=lambda kv: -sum(kv[1]))
+    for name, v in rows:
+        if a.match and a.match not in name:
+            continue
+        line = "%-60s %7d %10.2f %10.2f" % (name[:60], len(v), sum(v) / 1e3, sum(v) / len(v))
+        if have_pmc and name in fetch and name in write:
+            rb = sum(fetch[name]) / len(fetch[name]) * 1024.0 * ff
+            wb = sum(write[name]) / len(write[name]) * 1024.0 * fw
+            line += " %11.1f %11.1f %10.2f" % (rb / 1e6, wb / 1e6, (rb + wb) * len(v) / 1e9)
+        lines.append(line)
+    return "\n".join(lines)
+
+
 def read_traffic(path):
     """The json line of a committed traffic summary, or None."""
     try:
@@ -187,9 +235,17 @@ def main(argv=None):
         r.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
     r.add_argument("--copy-bytes", type=int, required=True)
     r.add_argument("--cells", type=int, required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("--trace", required=True)
+    for name in ("fetch", "write", "copy_fetch", "copy_write"):
+        k.add_argument("--" + name.replace("_", "-"), dest=name, default=None)
+    k.add_argument("--copy-bytes", type=int, default=None)
+    k.add_argument("--match", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         print(trace_summary(a.dir)[0])
+    elif a.cmd == "kernels":
+        print(kernels_summary(a))
     else:
         print(traffic_summary(a))
 
