@@ -278,7 +278,7 @@ class _BatchProblem(object):
                 p[n].grad = None
 
 
-def batched_gradients(hs, ninst, seed, ckpt_every, plain_sample, log):
+def batched_gradients(hs, ninst, seed, ckpt_every, plain_sample, log, use_graph=True):
     """Gradient evaluations per second (forward + loss + adjoint for every
     instance) of the batched path and of the plain per-instance path."""
     import torch
@@ -291,7 +291,8 @@ def batched_gradients(hs, ninst, seed, ckpt_every, plain_sample, log):
     log("batch: %d instances, %.3g cells, %.3g cell-steps per evaluation"
         % (ninst, cells, cell_steps))
     gb = hs.batch.GradBatch([x["shape"] for x in insts], BATCH_D[0],
-                            [x["nsteps"] for x in insts], [0] * ninst, ckpt_every=ckpt_every)
+                            [x["nsteps"] for x in insts], [0] * ninst, ckpt_every=ckpt_every,
+                            use_graph=use_graph)
     T0s = [p["T0"] for p in prob.params]
     # observations: the history of a perturbed "true" field plus 0.01 K noise
     rng = np.random.default_rng(seed + 1)
@@ -397,6 +398,9 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-seed", type=int, default=6)
     ap.add_argument("--ckpt-every", type=int, default=50)
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="launch the batched steps from the host instead of replaying graphs "
+                         "(for counter collection)")
     ap.add_argument("--plain-sample", type=int, default=0,
                     help="time the plain path on this many instances and scale (0: all)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -423,7 +427,7 @@ def main():
 
     if args.batch_only:
         res = batched_gradients(hs, args.batch_instances, args.batch_seed, args.ckpt_every,
-                                args.plain_sample, log)
+                                args.plain_sample, log, use_graph=not args.batch_no_graph)
         print(json.dumps(res), flush=True)
         return
 

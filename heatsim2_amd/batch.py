@@ -113,8 +113,7 @@ class GradBatch(object):
         srcs: per-instance source fields (W/m^3) or None.  Returns the list
         of per-instance surface histories; gradients flow to the
         coefficients and the source fields."""
-        hist = _BatchHistory.apply(self, *self._packed(coefs, T0s, srcs))
-        return self.unpack_hist(hist)
+        return list(_BatchHistory.apply(self, *self._packed(coefs, T0s, srcs)))
 
     def __del__(self):
         if getattr(self, "handle", None) is not None and _lib._lib is not None:
@@ -143,22 +142,29 @@ def _forward(gb, w, M, gz, gy, gx, T0, src):
 
 
 class _BatchHistory(torch.autograd.Function):
+    """Per-instance histories as separate outputs: their gradients arrive as
+    one tuple and are concatenated once (slices of one returned tensor would
+    each scatter a gradient the size of the whole history)."""
+
     @staticmethod
     def forward(ctx, gb, w, M, gz, gy, gx, T0, src):
         _, hist = _forward(gb, w, M, gz, gy, gx, T0, src)
         ctx.gb, ctx.coef, ctx.src, ctx.generation = gb, gb._coef, gb._src, gb._generation
         ctx.want_src = src is not None
-        return hist
+        return tuple(h.clone() for h in gb.unpack_hist(hist))
 
     @staticmethod
-    def backward(ctx, ghist):
+    def backward(ctx, *ghists):
         gb = ctx.gb
         if gb._generation != ctx.generation:
             raise RuntimeError("GradBatch: another forward ran since this history was computed; "
                                "backward needs the checkpoints of its own forward")
         L = _lib.load()
-        dev = ghist.device
-        ghist = ghist.to(torch.float64).contiguous()
+        dev = _lib.device()
+        ghist = torch.cat([(g if g is not None else
+                            torch.zeros((gb.nsteps[i],) + gb.shapes[i][1:], dtype=torch.float64,
+                                        device=dev)).reshape(-1).to(torch.float64)
+                           for i, g in enumerate(ghists)])
         gM, gz, gy, gx = (torch.empty(gb.N, dtype=torch.float64, device=dev) for _ in range(4))
         gsrc = torch.empty(gb.N, dtype=torch.float64, device=dev) if ctx.want_src else None
         work = gb.work()

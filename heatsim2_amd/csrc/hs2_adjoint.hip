@@ -117,6 +117,108 @@ __device__ __forceinline__ void adj_line_body(Grid g, V view, int CW, const doub
   }
 }
 
+// adj_line_body for x lines of per-voxel coefficients, staged through LDS:
+// the block's CW x-lines are consecutive in memory, so the right-hand side and
+// the x entries (M, g, w) are loaded cell by cell along the lines (coalesced)
+// into padded LDS rows; each thread then builds its segment's rows from LDS,
+// and the solution leaves through LDS the same way.  LDS: 6 P CW doubles of
+// reduced rows, then four arrays of CW rows of (L | 1) cells (R, M, g as
+// doubles, w as bytes).
+template <int SEG>
+__device__ __forceinline__ void adj_x_staged_body(Grid g, DenseView view, int CW, const double *R,
+                                                  int scale_m, double *U, int bx) {
+  extern __shared__ double lds[];
+  const int P = blockDim.x / CW;
+  const int L = g.nx, RW = g.nx | 1;
+  double *ra = lds;
+  double *rc = ra + 2 * P * CW;
+  double *rr = rc + 2 * P * CW;
+  double *sR = rr + 2 * P * CW;
+  double *sM = sR + CW * RW;
+  double *sG = sM + CW * RW;
+  uint8_t *sW = reinterpret_cast<uint8_t *>(sG + CW * RW);
+  const int nl = g.nz * g.ny, l0 = bx * CW;
+  const int nlb = min(CW, nl - l0);
+  const int ncell = nlb * L;
+  const long long c0 = (long long)l0 * L;
+  // UN cells per thread per batch, all loads issued before the first use
+  constexpr int UN = 8;
+  for (int e0 = 0; e0 < ncell; e0 += UN * (int)blockDim.x) {
+    double vr[UN], vm[UN], vg[UN];
+    uint8_t vw[UN];
+#pragma unroll
+    for (int u = 0; u < UN; u++) {
+      const long long c = c0 + min(e0 + u * (int)blockDim.x + (int)threadIdx.x, ncell - 1);
+      vr[u] = R[c];
+      vm[u] = view.M[c];
+      vg[u] = view.g[2][c];
+      vw[u] = view.w[c];
+    }
+#pragma unroll
+    for (int u = 0; u < UN; u++) {
+      const int e = e0 + u * (int)blockDim.x + (int)threadIdx.x;
+      if (e < ncell) {
+        const int lc = e / L, o = lc * RW + (e - lc * L);
+        sR[o] = vr[u];
+        sM[o] = vm[u];
+        sG[o] = vg[u];
+        sW[o] = vw[u];
+      }
+    }
+  }
+  __syncthreads();
+  const int cl = threadIdx.x % CW, p = threadIdx.x / CW;
+  const bool valid = cl < nlb;
+  const int row = (valid ? cl : 0) * RW;
+  const int t0 = p * SEG;
+  auto entry = [&](int t) {
+    const int o = row + min(max(t, 0), L - 1);
+    return Entry{sM[o], (double)sW[o], sG[o]};
+  };
+  double a[SEG], c[SEG], r[SEG];
+  Entry em = entry(t0 - 1);
+  Entry ec = entry(t0);
+#pragma unroll
+  for (int q = 0; q < SEG; q++) {
+    const int t = t0 + q;
+    const Entry ep = entry(t + 1);
+    const double gm = t >= 1 ? ec.g : 0.0;
+    const double gp = t + 1 <= L - 1 ? ep.g : 0.0;
+    const double b = ec.M + 0.5 * ec.w * (gm + gp);
+    const double inv = 1.0 / b;
+    const double Rq = sR[row + min(t, L - 1)];
+    const double rhs = scale_m ? ec.M * Rq : Rq;
+    const bool ok = valid && t < L;
+    a[q] = ok ? -0.5 * em.w * gm * inv : 0.0;
+    c[q] = ok ? -0.5 * ep.w * gp * inv : 0.0;
+    r[q] = ok ? rhs * inv : 0.0;
+    em = ec;
+    ec = ep;
+  }
+  double As, Cs, Rs, Ae, Ce, Re;
+  segment_reduce<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
+  ra[(2 * p) * CW + cl] = As; rc[(2 * p) * CW + cl] = Cs; rr[(2 * p) * CW + cl] = Rs;
+  ra[(2 * p + 1) * CW + cl] = Ae; rc[(2 * p + 1) * CW + cl] = Ce; rr[(2 * p + 1) * CW + cl] = Re;
+  __syncthreads();
+  // (every thread has built its rows from sR before the barriers in here,
+  // so sR can take the solution afterwards)
+  lds_pcr_ends(ra, rc, rr, p, P, CW, cl);
+  const double xs = rr[(2 * p) * CW + cl], xe = rr[(2 * p + 1) * CW + cl];
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < SEG; q++) {
+      const int t = t0 + q;
+      const double x = q == 0 ? xs : q == SEG - 1 ? xe : r[q] - a[q] * xs - c[q] * xe;
+      if (t < L) sR[row + t] = x;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ncell; e += blockDim.x) {
+    const int lc = e / L;
+    U[c0 + e] = sR[lc * RW + (e - lc * L)];
+  }
+}
+
 template <class V, int SEG, int AXIS>
 __global__ __launch_bounds__(256) void k_adj_line(Grid g, V view, int CW, const double *R,
                                                   int scale_m, double *U) {
@@ -126,21 +228,17 @@ __global__ __launch_bounds__(256) void k_adj_line(Grid g, V view, int CW, const 
 }
 
 // ------------------------------------------------------ adjoint stencil
-// One thread per cell.  Writes lam_out = dJ/dT^n (plus the readout adjoint
-// of this step's input field when ghist != null: the insulating-top readout
-// T0 - 5/16 (T1 - T0) of T^n), accumulates the sensitivities into gM and
-// gg[axis] (the cell's minus face on that axis), and writes gsrc = w u_x
-// when gsrc != null.
+// dJ/dT^n of cell c = (k, j, i) (plus the readout adjoint of this step's input
+// field when ghist != null: the insulating-top readout T0 - 5/16 (T1 - T0) of
+// T^n), returned; accumulates the cell's sensitivities into gM and gg[axis]
+// (the cell's minus face on that axis), and writes gsrc = w u_x when gsrc !=
+// null.  `lds`: the view's tables (all axes).
 template <class V>
-__device__ __forceinline__ void adj_stencil_body(Grid g, V view, const AdjFields &f, int blk) {
-  extern __shared__ double lds[];
-  view.load_tables(lds, -1);
-  __syncthreads();
+__device__ __forceinline__ double adj_stencil_cell(Grid g, const V &view, const double *lds,
+                                                   const AdjFields &f, int c, int k, int j,
+                                                   int i) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const int sxy = nx * ny, n = sxy * nz;
-  const int c = blk * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const int k = c / sxy, rem = c - k * sxy, j = rem / nx, i = rem - j * nx;
+  const int sxy = nx * ny;
   const int coord[3] = {k, j, i};
   const int len[3] = {nz, ny, nx};
   const int step[3] = {sxy, nx, 1};
@@ -186,12 +284,27 @@ __device__ __forceinline__ void adj_stencil_body(Grid g, V view, const AdjFields
     }
   }
   if (f.ghist) {
+    const int rem = c - k * sxy;
     if (k == 0) lam += (21.0 / 16.0) * f.ghist[rem];
     else if (k == 1) lam -= (5.0 / 16.0) * f.ghist[rem];
   }
-  f.lam_out[c] = lam;
   f.gM[c] -= uxc * (Txc - Tnc) + uyc * (Tyc - Txc) + uzc * (T1c - Tyc);
   if (f.gsrc) f.gsrc[c] = wc * uxc;
+  return lam;
+}
+
+// One thread per cell: lam_out = adj_stencil_cell.
+template <class V>
+__device__ __forceinline__ void adj_stencil_body(Grid g, V view, const AdjFields &f, int blk) {
+  extern __shared__ double lds[];
+  view.load_tables(lds, -1);
+  __syncthreads();
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int sxy = nx * ny, n = sxy * nz;
+  const int c = blk * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int k = c / sxy, rem = c - k * sxy, j = rem / nx, i = rem - j * nx;
+  f.lam_out[c] = adj_stencil_cell<V>(g, view, lds, f, c, k, j, i);
 }
 
 template <class V>
@@ -306,19 +419,25 @@ hipError_t launch_fill(double *a, size_t n, double v, hipStream_t s) {
 
 // ------------------------------------------------------ batched adjoint
 template <int SEG, int AXIS>
-__global__ __launch_bounds__(kBatchCW *kBatchMaxLen / kBatchSeg) void kb_adj_line(
-    BatchMap m, DenseView v, const double *R, int scale_m, double *U) {
+__global__ __launch_bounds__(kBatchThreads) void kb_adj_line(BatchMap m, DenseView v,
+                                                             const double *R, int scale_m,
+                                                             double *U) {
   BatchInst I;
   int local, t;
   if (!batch_locate(m, I, local, t)) return;
+  const int cw = I.cw[AXIS];
   int bx = local, by = 0;
   if (AXIS != 2) {
-    const int nbx = (I.nx + kBatchCW - 1) / kBatchCW;
+    const int nbx = (I.nx + cw - 1) / cw;
     bx = local % nbx;
     by = local / nbx;
   }
-  adj_line_body<DenseView, SEG, AXIS>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), kBatchCW,
-                                      R + I.off, scale_m, U + I.off, bx, by);
+  if constexpr (AXIS == 2)
+    adj_x_staged_body<SEG>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), cw, R + I.off, scale_m,
+                           U + I.off, bx);
+  else
+    adj_line_body<DenseView, SEG, AXIS>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), cw,
+                                        R + I.off, scale_m, U + I.off, bx, by);
 }
 
 // f: base pointers of the concatenated fields; f.ghist: base of the
@@ -336,21 +455,6 @@ __global__ __launch_bounds__(256) void kb_adj_stencil(BatchMap m, DenseView v, A
   g.ghist = (f.ghist && t >= 1) ? f.ghist + I.hoff + (long long)(t - 1) * I.ny * I.nx : nullptr;
   g.gsrc = (f.gsrc && t == I.src_step) ? f.gsrc + o : nullptr;
   adj_stencil_body<DenseView>(Grid{I.nz, I.ny, I.nx}, offset_view(v, o), g, local);
-}
-
-// forward: before step t, T^t -> checkpoint slot ceil((n-t)/K)-1 when
-// (n - t) % K == 0 or t == 0
-__global__ void kb_ckpt_store(BatchMap m, const double *T, double *ck) {
-  BatchInst I;
-  int local, t;
-  if (!batch_locate(m, I, local, t)) return;
-  const int r = I.nsteps - t;
-  if (r % m.K != 0 && t != 0) return;
-  const long long n = (long long)I.nz * I.ny * I.nx;
-  const long long c = (long long)local * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const int slot = (r + m.K - 1) / m.K - 1;
-  ck[I.ckoff + slot * n + c] = T[I.off + c];
 }
 
 // backward segment start: checkpoint of the segment -> segment slot K - (e - s)
@@ -387,9 +491,13 @@ __global__ void kb_counter(int *ctr, int which, int value, int add) {
 
 hipError_t launch_batch_adj_line(const BatchMap &m, int nblocks, int axis, const DenseView &v,
                                  const double *R, int scale_m, double *U, hipStream_t s) {
-  constexpr int P = kBatchMaxLen / kBatchSeg;
-  const size_t shm = sizeof(double) * 6 * (size_t)P * kBatchCW;
-  const dim3 grid(nblocks), block(P * kBatchCW);
+  // x lines: blocks of kBatchXThreads staging CW rows of (L | 1) cells,
+  // CW L <= kBatchXThreads kBatchSeg
+  const int threads = axis == 2 ? kBatchXThreads : kBatchThreads;
+  const size_t rows = (size_t)kBatchXThreads * kBatchSeg + kBatchXThreads;
+  const size_t shm = sizeof(double) * 6 * (size_t)threads +
+                     (axis == 2 ? (3 * sizeof(double) + 1) * rows : 0);
+  const dim3 grid(nblocks), block(threads);
   switch (axis) {
     case 0: hipLaunchKernelGGL((kb_adj_line<kBatchSeg, 0>), grid, block, shm, s, m, v, R, scale_m, U); break;
     case 1: hipLaunchKernelGGL((kb_adj_line<kBatchSeg, 1>), grid, block, shm, s, m, v, R, scale_m, U); break;
@@ -401,12 +509,6 @@ hipError_t launch_batch_adj_line(const BatchMap &m, int nblocks, int axis, const
 hipError_t launch_batch_adj_stencil(const BatchMap &m, int nblocks, const DenseView &v,
                                     const AdjFields &f, hipStream_t s) {
   hipLaunchKernelGGL(kb_adj_stencil, dim3(nblocks), dim3(256), 0, s, m, v, f);
-  return hipGetLastError();
-}
-
-hipError_t launch_batch_ckpt_store(const BatchMap &m, int nblocks, const double *T, double *ck,
-                                   hipStream_t s) {
-  hipLaunchKernelGGL(kb_ckpt_store, dim3(nblocks), dim3(256), 0, s, m, T, ck);
   return hipGetLastError();
 }
 

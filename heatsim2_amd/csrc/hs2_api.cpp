@@ -299,12 +299,11 @@ hs2::BatchMap batch_map(const hs2_batch *b, int kind, int rule, int j) {
 }
 
 int blocks_of(const hs2::BatchInst &I, int kind) {
-  const int nbx = (I.nx + hs2::kBatchCW - 1) / hs2::kBatchCW;
   switch (kind) {
     case KIND_X2: return (I.nz * I.ny + hs2::kBatchXLines - 1) / hs2::kBatchXLines;
-    case KIND_Y: return nbx * I.nz;
-    case KIND_Z: return nbx * I.ny;
-    case KIND_AX: return (I.nz * I.ny + hs2::kBatchCW - 1) / hs2::kBatchCW;
+    case KIND_Y: return (I.nx + I.cw[1] - 1) / I.cw[1] * I.nz;
+    case KIND_Z: return (I.nx + I.cw[0] - 1) / I.cw[0] * I.ny;
+    case KIND_AX: return (I.nz * I.ny + I.cw[2] - 1) / I.cw[2];
     default: return (int)(((long long)I.nz * I.ny * I.nx + 255) / 256);
   }
 }
@@ -324,11 +323,11 @@ void batch_free(hs2_batch *b) {
 }
 
 // work layout: ws (N) | checkpoints (CK) | segment T^t (K+1 N) | T* (K N) | T** (K N) |
-// ux uy uz lam0 lam1 (5 N)
+// ux uy uz lam (4 N)
 struct BatchWork {
-  double *ws, *ck, *seg, *segx, *segy, *ux, *uy, *uz, *lam[2];
+  double *ws, *ck, *seg, *segx, *segy, *ux, *uy, *uz, *lam;
 };
-long long batch_work_doubles(const hs2_batch *b) { return b->N * (3LL * b->K + 7) + b->CK; }
+long long batch_work_doubles(const hs2_batch *b) { return b->N * (3LL * b->K + 6) + b->CK; }
 BatchWork batch_work(const hs2_batch *b, double *w) {
   BatchWork r;
   r.ws = w;
@@ -339,8 +338,7 @@ BatchWork batch_work(const hs2_batch *b, double *w) {
   r.ux = r.segy + (size_t)b->K * b->N;
   r.uy = r.ux + b->N;
   r.uz = r.uy + b->N;
-  r.lam[0] = r.uz + b->N;
-  r.lam[1] = r.lam[0] + b->N;
+  r.lam = r.uz + b->N;
   return r;
 }
 
@@ -349,11 +347,8 @@ hipError_t emit_forward(hs2_batch *b, double *T, const double *src, double *hist
                         const BatchWork &w, int steps, hipStream_t s) {
   hipError_t e = hipSuccess;
   for (int c = 0; c < steps && e == hipSuccess; c++) {
-    e = hs2::launch_batch_ckpt_store(batch_map(b, KIND_CELL, hs2::RULE_FWD, 0), b->nblk[KIND_CELL],
-                                     T, w.ck, s);
-    if (e == hipSuccess)
-      e = hs2::launch_batch_x2(batch_map(b, KIND_X2, hs2::RULE_FWD, 0), b->nblk[KIND_X2], b->dv, T,
-                               w.ws, src, s);
+    e = hs2::launch_batch_x2(batch_map(b, KIND_X2, hs2::RULE_FWD, 0), b->nblk[KIND_X2], b->dv, T,
+                             w.ws, src, w.ck, s);
     if (e == hipSuccess)
       e = hs2::launch_batch_strided(batch_map(b, KIND_Y, hs2::RULE_FWD, 0), b->nblk[KIND_Y], 1,
                                     b->dv, T, w.ws, w.ws, nullptr, s);
@@ -378,7 +373,7 @@ hipError_t emit_backward_segment(hs2_batch *b, const double *src, const double *
     const double *in = w.seg + (size_t)j * N;
     double *tx = w.segx + (size_t)j * N, *ty = w.segy + (size_t)j * N;
     e = hs2::launch_batch_x2(batch_map(b, KIND_X2, hs2::RULE_RECOMP, j), b->nblk[KIND_X2], b->dv,
-                             in, tx, src, s);
+                             in, tx, src, nullptr, s);
     if (e == hipSuccess)
       e = hs2::launch_batch_strided(batch_map(b, KIND_Y, hs2::RULE_RECOMP, j), b->nblk[KIND_Y], 1,
                                     b->dv, in, tx, ty, nullptr, s);
@@ -386,10 +381,12 @@ hipError_t emit_backward_segment(hs2_batch *b, const double *src, const double *
       e = hs2::launch_batch_strided(batch_map(b, KIND_Z, hs2::RULE_RECOMP, j), b->nblk[KIND_Z], 0,
                                     b->dv, in, ty, w.seg + (size_t)(j + 1) * N, nullptr, s);
   }
+  // adjoint steps, latest first: z, y, x transposed line solves from
+  // lam = dJ/dT^{t+1}, then the stencil, which leaves dJ/dT^t in lam
   for (int j = 0; j < K && e == hipSuccess; j++) {
     const size_t q = (size_t)(K - 1 - j);
     e = hs2::launch_batch_adj_line(batch_map(b, KIND_Z, hs2::RULE_BWD, j), b->nblk[KIND_Z], 0,
-                                   b->dv, w.lam[j & 1], 0, w.uz, s);
+                                   b->dv, w.lam, 0, w.uz, s);
     if (e == hipSuccess)
       e = hs2::launch_batch_adj_line(batch_map(b, KIND_Y, hs2::RULE_BWD, j), b->nblk[KIND_Y], 1,
                                      b->dv, w.uz, 1, w.uy, s);
@@ -402,7 +399,7 @@ hipError_t emit_backward_segment(hs2_batch *b, const double *src, const double *
       f.Tn1 = w.seg + (q + 1) * N;
       f.ux = w.ux; f.uy = w.uy; f.uz = w.uz;
       f.ghist = ghist;
-      f.lam_out = w.lam[(j + 1) & 1];
+      f.lam_out = w.lam;
       f.gM = gM;
       f.gg[0] = gz; f.gg[1] = gy; f.gg[2] = gx;
       f.gsrc = gsrc;
@@ -1070,6 +1067,10 @@ int hs2_batch_create(const hs2_batch_inst *in, int ninst, int ckpt_every, hs2_ba
     hs2::BatchInst I{};
     I.nz = s.nz; I.ny = s.ny; I.nx = s.nx; I.nsteps = s.nsteps;
     I.src_step = s.src_step; I.dz = s.dz;
+    I.xseg = hs2::batch_x_seg(s.nx);
+    I.cw[0] = hs2::batch_line_cw(s.nz);
+    I.cw[1] = hs2::batch_line_cw(s.ny);
+    I.cw[2] = hs2::batch_line_cw(s.nx, hs2::kBatchXThreads);
     const long long n = (long long)s.nz * s.ny * s.nx;
     I.off = off;
     I.hoff = hoff;
@@ -1215,7 +1216,7 @@ int hs2_batch_backward(hs2_batch *b, const double *d_src, const double *d_grad_h
   HIP_TRY(hs2::launch_fill(d_gx, b->N, 0.0, s));
   if (d_gsrc) HIP_TRY(hs2::launch_fill(d_gsrc, b->N, 0.0, s));
   HIP_TRY(hs2::launch_batch_lam_init(batch_map(b, KIND_CELL, hs2::RULE_ALL, 0), b->nblk[KIND_CELL],
-                                     d_grad_hist, w.lam[0], s));
+                                     d_grad_hist, w.lam, s));
   HIP_TRY(hs2::launch_batch_counter(b->d_ctr, 1, 0, 0, s));
   const int reps = (b->maxsteps + b->K - 1) / b->K;
   const std::vector<const void *> key = {d_src, d_grad_hist, d_gM, d_gz, d_gy, d_gx, d_gsrc,

@@ -20,10 +20,22 @@
 namespace hs2 {
 
 constexpr int kBatchMaxLen = 128;  // axis length limit of the batched kernels
-constexpr int kBatchCW = 16;       // lines per block of the strided / line kernels
-constexpr int kBatchSeg = 8;       // cells per thread segment (16 segments x 8 = 128)
-constexpr int kBatchXLines = 8;    // x-lines per block of the batched x sweep
+constexpr int kBatchThreads = 256; // threads per block of the strided / line kernels
+constexpr int kBatchXThreads = 128; // threads per block of the adjoint x-line kernel
+constexpr int kBatchSeg = 8;       // cells per thread segment of the line kernels
+constexpr int kBatchXLines = 8;    // x-lines per block of the batched x sweep (16 lanes each)
 constexpr int kBatchAlign = 32;    // instance offsets are multiples of this many cells
+
+// Lines side by side per block of a line kernel on lines of L cells: the
+// block's 256 threads hold P = 256/cw segments of kBatchSeg cells per line,
+// the fewest that cover L (each instance sized to its own axes).
+__host__ __device__ inline int batch_line_cw(int L, int threads = kBatchThreads) {
+  int p = 1;
+  while (p * kBatchSeg < L) p <<= 1;
+  return threads / p;
+}
+// cells per lane (16 lanes per line) of the batched x sweep's solve
+__host__ __device__ inline int batch_x_seg(int nx) { return nx <= 32 ? 2 : nx <= 64 ? 4 : 8; }
 
 struct BatchInst {
   int nz, ny, nx, nsteps;
@@ -31,6 +43,9 @@ struct BatchInst {
   long long hoff;   // first value of the surface history (nsteps x ny x nx)
   long long ckoff;  // first value of the checkpoints (ceil(nsteps/K) fields)
   int src_step;     // step whose source field applies (-1: none)
+  int xseg;         // batch_x_seg(nx)
+  int cw[3];        // batch_line_cw of the z, y, x line lengths (x: of the
+                    // adjoint x-line kernel's kBatchXThreads)
   int pad;
   double dz;
 };
@@ -45,12 +60,25 @@ struct BatchMap {
   int rule, j, K;
 };
 
+// Logical block of hardware block b among nb.  Hardware hands block b to XCD
+// b % 8, each with its own L2; consecutive logical blocks touch neighbouring
+// rows (and share cache lines of rows that do not start on a line), so they
+// go to one XCD in chunks of kBatchChunk, and the chunks go round the XCDs so
+// that every XCD keeps a share of every instance (instances finish at
+// different steps).  Blocks past the last whole round of chunks keep their
+// index.
+constexpr int kBatchChunk = 128;
+__device__ __forceinline__ int batch_block(int b, int nb) {
+  constexpr int R = 8 * kBatchChunk;
+  if (b >= nb / R * R) return b;
+  const int x = b & 7, k = b >> 3;
+  return ((k / kBatchChunk) * 8 + x) * kBatchChunk + k % kBatchChunk;
+}
+
 // Instance, local block and step of this block; false if the instance has
 // nothing to do under the launch's rule (the whole block returns).
 __device__ __forceinline__ bool batch_locate(const BatchMap &m, BatchInst &I, int &local, int &t) {
-  // (no XCD regrouping here: instances finish at different steps, and whole
-  // instances per XCD would leave XCDs idle)
-  const int gb = blockIdx.x;
+  const int gb = batch_block(blockIdx.x, gridDim.x);
   const int i = m.blk_inst[gb];
   local = gb - m.blk0[i];
   I = m.inst[i];
@@ -91,8 +119,10 @@ __device__ __forceinline__ DenseView offset_view(DenseView v, long long off) {
 namespace hs2 {
 struct AdjFields;
 // launchers (hs2_kernels.hip, hs2_adjoint.hip); nblocks = this kind's block total
+// the x sweep; with ck != nullptr (forward) a block first copies its lines of
+// T^t into the instance's checkpoint slot on a checkpoint step
 hipError_t launch_batch_x2(const BatchMap &m, int nblocks, const DenseView &v, const double *Tn,
-                           double *Ts, const double *src, hipStream_t s);
+                           double *Ts, const double *src, double *ck, hipStream_t s);
 hipError_t launch_batch_strided(const BatchMap &m, int nblocks, int axis, const DenseView &v,
                                 const double *Tn, const double *Y, double *X, double *hist,
                                 hipStream_t s);
@@ -100,8 +130,6 @@ hipError_t launch_batch_adj_line(const BatchMap &m, int nblocks, int axis, const
                                  const double *R, int scale_m, double *U, hipStream_t s);
 hipError_t launch_batch_adj_stencil(const BatchMap &m, int nblocks, const DenseView &v,
                                     const AdjFields &f, hipStream_t s);
-hipError_t launch_batch_ckpt_store(const BatchMap &m, int nblocks, const double *T, double *ck,
-                                   hipStream_t s);
 hipError_t launch_batch_ckpt_load(const BatchMap &m, int nblocks, const double *ck, double *seg,
                                   long long N, hipStream_t s);
 hipError_t launch_batch_lam_init(const BatchMap &m, int nblocks, const double *ghist, double *lam,

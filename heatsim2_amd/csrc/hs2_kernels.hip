@@ -169,6 +169,14 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   // BUF: phase 1's x-class words, one per quad, for phase 2 (which would
   // otherwise fetch them again right after the barrier)
   [[maybe_unused]] uint32_t *cbuf = reinterpret_cast<uint32_t *>(rbuf + LPB * RS);
+  // PARK (per-voxel coefficients): phase 1 also leaves each cell's x entry
+  // (M, g, w) in LDS beside its right-hand side, so that phase 2, whose lanes
+  // own runs of SEG cells, reads them there instead of from memory in
+  // SEG-strided (uncoalesced) loads
+  constexpr bool PARK = std::is_same<V, DenseView>::value;
+  [[maybe_unused]] double *pM = rbuf + LPB * RS;
+  [[maybe_unused]] double *pG = pM + LPB * RS;
+  [[maybe_unused]] uint8_t *pW = reinterpret_cast<uint8_t *>(pG + LPB * RS);
   if (SRC && vmap)
     for (int t = threadIdx.x; t < kMaxVtab; t += NT) vtab[t] = vtab_g[t];
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -445,9 +453,17 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           out[q] = ex[q].M * T0 + ex[q].w * expl;
         }
         if (FQ || li[b] < LPB) {
-          double *dst = rbuf + li[b] * RS + x0 + (x0 >> 4);
+          const int o = li[b] * RS + x0 + (x0 >> 4);
 #pragma unroll
-          for (int q = 0; q < 4; q++) dst[q] = out[q];
+          for (int q = 0; q < 4; q++) rbuf[o + q] = out[q];
+          if constexpr (PARK) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              pM[o + q] = rx[b][q].M;
+              pG[o + q] = rx[b][q].g;
+              pW[o + q] = (uint8_t)rx[b][q].w;
+            }
+          }
         }
       }
     };
@@ -514,7 +530,15 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           if (vmap) expl += vtab[vmap[lbase[lb][0] + x]];
           if (sfield) expl += sfield[lbase[lb][0] + x];
         }
-        if (li[b] < LPB) rbuf[li[b] * RS + x + (x >> 4)] = e.M * T0 + e.w * expl;
+        if (li[b] < LPB) {
+          const int o = li[b] * RS + x + (x >> 4);
+          rbuf[o] = e.M * T0 + e.w * expl;
+          if constexpr (PARK) {
+            pM[o] = rx[b].M;
+            pG[o] = rx[b].g;
+            pW[o] = (uint8_t)rx[b].w;
+          }
+        }
       }
     }
   }
@@ -526,7 +550,14 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   const int base = lbase[lq][0];
   const int i0 = lig * SEG;
   typename V::Raw rxs[SEG];
-  if constexpr (BUF) {
+  if constexpr (PARK) {
+#pragma unroll
+    for (int q = 0; q < SEG; q++) {
+      const int x = min(i0 + q, nx - 1);
+      const int o = lq * RS + x + (x >> 4);
+      rxs[q] = DenseRaw{pM[o], pG[o], pW[o]};
+    }
+  } else if constexpr (BUF) {
     static_assert(SEG % 4 == 0, "BUF path reads class words");
     const uint32_t *cw = cbuf + lq * (SEG * G / 4) + i0 / 4;
 #pragma unroll
@@ -895,10 +926,12 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *T
                             : (nlines + LPB - 1) / LPB;
   constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;
   const bool src = vmap || sfield;
-  // BUF variants (class maps, SEG % 4 == 0) also park one class word per quad
+  // BUF variants (class maps, SEG % 4 == 0) also park one class word per quad,
+  // per-voxel variants each cell's x entry (M, g, w)
   const size_t shm =
       sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + (src ? kMaxVtab : 0) + LPB * RS) +
-      (std::is_same<V, ClassView>::value && SEG % 4 == 0 ? (size_t)LPB * SEG * G : 0);
+      (std::is_same<V, ClassView>::value && SEG % 4 == 0 ? (size_t)LPB * SEG * G : 0) +
+      (std::is_same<V, DenseView>::value ? (2 * sizeof(double) + 1) * (size_t)LPB * RS : 0);
   const bool full = x2_full(g, SEG * G, LPB, ty);
   if constexpr (std::is_same<V, ClassView>::value && SEG % 4 == 0) {
     // 16-cell lanes, 16 x 16 in 8 x 2 tiles or 32 x 16 in 2 x 2 tiles: the
@@ -1194,52 +1227,85 @@ hipError_t launch_surface(const double *T, int ny, int nx, double dz, double *ou
 // ------------------------------------------------------ batched sweeps
 // One launch covers every instance of a batch (per-voxel coefficients,
 // axis lengths <= kBatchMaxLen); see hs2_batch.h for the block mapping.
-template <int SEG, int G, int LPB>
+// Lines of T^t of one x-sweep block (LPB whole x-lines, contiguous) copied
+// into the instance's checkpoint slot when step t starts a checkpoint
+// segment: (nsteps - t) % K == 0, or t == 0 (the segments are aligned to
+// the instance's end).
+__device__ __forceinline__ void batch_ckpt_lines(const BatchMap &m, const BatchInst &I, int t,
+                                                 int local, int lpb, const double *T,
+                                                 double *ck) {
+  const int r = I.nsteps - t;
+  if (r % m.K != 0 && t != 0) return;
+  const long long n = (long long)I.nz * I.ny * I.nx;
+  const int slot = (r + m.K - 1) / m.K - 1;
+  const long long c0 = (long long)local * lpb * I.nx;
+  const long long c1 = min(n, c0 + (long long)lpb * I.nx);
+  double *dst = ck + I.ckoff + (long long)slot * n;
+  for (long long c = c0 + threadIdx.x; c < c1; c += blockDim.x) dst[c] = T[I.off + c];
+}
+
+template <int G, int LPB>
 __global__ __launch_bounds__(G *LPB) void kb_sweep_x2(BatchMap m, DenseView v, const double *Tn,
-                                                     double *Ts, const double *src) {
+                                                     double *Ts, const double *src, double *ck) {
   BatchInst I;
   int local, t;
   if (!batch_locate(m, I, local, t)) return;
+  if (ck) batch_ckpt_lines(m, I, t, local, LPB, Tn, ck);
   const double *sf = (src && t == I.src_step) ? src + I.off : nullptr;
-  sweep_x2_body<DenseView, SEG, G, LPB, true>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off),
-                                              Tn + I.off, Ts + I.off, nullptr, nullptr, sf,
-                                              local, 0);
+  const Grid g{I.nz, I.ny, I.nx};
+  const DenseView vi = offset_view(v, I.off);
+  // lanes hold 2, 4 or 8 cells: 16 lanes cover the instance's x lines
+  switch (I.xseg) {
+    case 2:
+      sweep_x2_body<DenseView, 2, G, LPB, true>(g, vi, Tn + I.off, Ts + I.off, nullptr, nullptr,
+                                                sf, local, 0);
+      break;
+    case 4:
+      sweep_x2_body<DenseView, 4, G, LPB, true>(g, vi, Tn + I.off, Ts + I.off, nullptr, nullptr,
+                                                sf, local, 0);
+      break;
+    default:
+      sweep_x2_body<DenseView, 8, G, LPB, true>(g, vi, Tn + I.off, Ts + I.off, nullptr, nullptr,
+                                                sf, local, 0);
+      break;
+  }
 }
 
 template <int SEG, int AXIS>
-__global__ __launch_bounds__(kBatchCW *kBatchMaxLen / kBatchSeg) void kb_sweep_strided(
+__global__ __launch_bounds__(kBatchThreads) void kb_sweep_strided(
     BatchMap m, DenseView v, const double *Tn, const double *Y, double *X, double *hist) {
   BatchInst I;
   int local, t;
   if (!batch_locate(m, I, local, t)) return;
-  const int nbx = (I.nx + kBatchCW - 1) / kBatchCW;
+  const int cw = I.cw[AXIS];
+  const int nbx = (I.nx + cw - 1) / cw;
   double *surf = (AXIS == 0 && hist) ? hist + I.hoff + (long long)t * I.ny * I.nx : nullptr;
-  sweep_strided_body<DenseView, SEG, AXIS>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off),
-                                           kBatchCW, Tn + I.off, Y + I.off, X + I.off, surf, I.dz,
+  sweep_strided_body<DenseView, SEG, AXIS>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), cw,
+                                           Tn + I.off, Y + I.off, X + I.off, surf, I.dz,
                                            local % nbx, local / nbx);
 }
 
 hipError_t launch_batch_x2(const BatchMap &m, int nblocks, const DenseView &v, const double *Tn,
-                           double *Ts, const double *src, hipStream_t s) {
-  constexpr int SEG = kBatchMaxLen / 16, G = 16, LPB = kBatchXLines;
-  constexpr int RS = G * SEG + ((G * SEG) >> 4) + 1;
-  const size_t shm = sizeof(double) * (kMaxVtab + LPB * RS);
-  hipLaunchKernelGGL((kb_sweep_x2<SEG, G, LPB>), dim3(nblocks), dim3(G * LPB), shm, s, m, v, Tn,
-                     Ts, src);
+                           double *Ts, const double *src, double *ck, hipStream_t s) {
+  constexpr int G = 16, LPB = kBatchXLines, SEGMAX = kBatchMaxLen / 16;
+  constexpr int RS = G * SEGMAX + ((G * SEGMAX) >> 4) + 1;
+  // vtab, right-hand sides, parked x entries (M, g as doubles, w as bytes)
+  const size_t shm = sizeof(double) * (kMaxVtab + 3 * LPB * RS) + (size_t)LPB * RS;
+  hipLaunchKernelGGL((kb_sweep_x2<G, LPB>), dim3(nblocks), dim3(G * LPB), shm, s, m, v, Tn, Ts,
+                     src, ck);
   return hipGetLastError();
 }
 
 hipError_t launch_batch_strided(const BatchMap &m, int nblocks, int axis, const DenseView &v,
                                 const double *Tn, const double *Y, double *X, double *hist,
                                 hipStream_t s) {
-  constexpr int P = kBatchMaxLen / kBatchSeg;
-  const size_t shm = sizeof(double) * 6 * (size_t)P * kBatchCW;
+  const size_t shm = sizeof(double) * 6 * (size_t)kBatchThreads;
   if (axis == 1)
-    hipLaunchKernelGGL((kb_sweep_strided<kBatchSeg, 1>), dim3(nblocks), dim3(P * kBatchCW), shm, s,
-                       m, v, Tn, Y, X, hist);
+    hipLaunchKernelGGL((kb_sweep_strided<kBatchSeg, 1>), dim3(nblocks), dim3(kBatchThreads), shm,
+                       s, m, v, Tn, Y, X, hist);
   else
-    hipLaunchKernelGGL((kb_sweep_strided<kBatchSeg, 0>), dim3(nblocks), dim3(P * kBatchCW), shm, s,
-                       m, v, Tn, Y, X, hist);
+    hipLaunchKernelGGL((kb_sweep_strided<kBatchSeg, 0>), dim3(nblocks), dim3(kBatchThreads), shm,
+                       s, m, v, Tn, Y, X, hist);
   return hipGetLastError();
 }
 
