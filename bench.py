@@ -278,6 +278,47 @@ class _BatchProblem(object):
                 p[n].grad = None
 
 
+def batched_kernels(hs, ninst, seed, ckpt_every, log, use_graph=True, reps=1):
+    """The batched library calls alone (hs2_batch_forward, then
+    hs2_batch_backward of a fixed history gradient), with no autograd or
+    torch arithmetic in between: the run that per-kernel counters are
+    collected on.  Returns seconds per forward and per backward."""
+    import ctypes
+    import torch
+    from heatsim2_amd import _lib, batch
+    dev = _lib.device()
+    L = _lib.load()
+    insts = batch_workload(ninst, seed)
+    prob = _BatchProblem(hs, insts, dev)
+    gb = hs.batch.GradBatch([x["shape"] for x in insts], BATCH_D[0],
+                            [x["nsteps"] for x in insts], [0] * ninst, ckpt_every=ckpt_every,
+                            use_graph=use_graph)
+    with torch.no_grad():
+        packed = gb._packed([prob.coef(i) for i in range(ninst)],
+                            [p["T0"] for p in prob.params],
+                            [prob.src(i) for i in range(ninst)])
+        ghist = torch.full((gb.H,), 1e-3, dtype=torch.float64, device=dev)
+        grads = [torch.empty(gb.N, dtype=torch.float64, device=dev) for _ in range(5)]
+        tf = tb = 0.0
+        for rep in range(reps + 1):  # the first pass captures the graphs
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            batch._forward(gb, *packed)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            _lib.check("hs2_batch_backward", L.hs2_batch_backward(
+                gb.handle, _lib.ptr(gb._src), _lib.ptr(ghist),
+                *[_lib.ptr(g) for g in grads], _lib.ptr(gb.work()),
+                ctypes.c_size_t(gb.work_doubles), _lib.stream_handle()))
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if rep:
+                tf += (t1 - t0) / reps
+                tb += (t2 - t1) / reps
+    log("batched library calls: forward %.3f s, backward %.3f s" % (tf, tb))
+    return {"forward_s": tf, "backward_s": tb, "instances": ninst}
+
+
 def batched_gradients(hs, ninst, seed, ckpt_every, plain_sample, log, use_graph=True):
     """Gradient evaluations per second (forward + loss + adjoint for every
     instance) of the batched path and of the plain per-instance path."""
@@ -395,6 +436,8 @@ def main():
     ap.add_argument("--no-strong", action="store_true", help="skip the strongly coupled step")
     ap.add_argument("--strong-steps", type=int, default=200)
     ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--batch-kernels", action="store_true",
+                    help="only the batched forward and backward library calls (for counters)")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-seed", type=int, default=6)
     ap.add_argument("--ckpt-every", type=int, default=50)
@@ -425,6 +468,11 @@ def main():
     from heatsim2_amd import _lib
     from hs2_cases import cases, to_package_args
 
+    if args.batch_kernels:
+        res = batched_kernels(hs, args.batch_instances, args.batch_seed, args.ckpt_every, log,
+                              use_graph=not args.batch_no_graph)
+        print(json.dumps(res), flush=True)
+        return
     if args.batch_only:
         res = batched_gradients(hs, args.batch_instances, args.batch_seed, args.ckpt_every,
                                 args.plain_sample, log, use_graph=not args.batch_no_graph)

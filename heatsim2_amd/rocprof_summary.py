@@ -199,6 +199,8 @@ def kernels_summary(a):
     hdr = "%-60s %7s %10s %10s" % ("kernel", "calls", "total_ms", "mean_us")
     if have_pmc:
         hdr += " %11s %11s %10s" % ("read_MB", "write_MB", "total_GB")
+        if a.cells:
+            hdr += " %8s" % "B/cell"
     lines.append(hdr)
     rows = sorted(dur.items(), key=lambda kv: -sum(kv[1]))
     for name, v in rows:
@@ -209,6 +211,8 @@ def kernels_summary(a):
             rb = sum(fetch[name]) / len(fetch[name]) * 1024.0 * ff
             wb = sum(write[name]) / len(write[name]) * 1024.0 * fw
             line += " %11.1f %11.1f %10.2f" % (rb / 1e6, wb / 1e6, (rb + wb) * len(v) / 1e9)
+            if a.cells:
+                line += " %8.1f" % ((rb + wb) / a.cells)
         lines.append(line)
     return "\n".join(lines)
 
@@ -237,6 +241,8 @@ def main(argv=None):
     r.add_argument("--cells", type=int, required=True)
     k = sub.add_parser("kernels")
     k.add_argument("--trace", required=True)
+    k.add_argument("--cells", type=float, default=None,
+                   help="mean cells one launch works on: adds HBM bytes per cell")
     for name in ("fetch", "write", "copy_fetch", "copy_write"):
         k.add_argument("--" + name.replace("_", "-"), dest=name, default=None)
     k.add_argument("--copy-bytes", type=int, default=None)
