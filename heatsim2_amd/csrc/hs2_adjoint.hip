@@ -39,21 +39,38 @@ namespace hs2 {
 // Axes 1, 0: columns i = bx*CW + cl, by = k (y) or j (z).  A block has P
 // thread segments of SEG cells per line; P = blockDim.x / CW.  U must not
 // alias R.
-template <class V, int SEG, int AXIS>
+// FLAT (batched, axes 0 and 1): columns as in sweep_strided_body's FLAT, by
+// = P segments per line.
+template <class V, int SEG, int AXIS, bool FLAT = false>
 __device__ __forceinline__ void adj_line_body(Grid g, V view, int CW, const double *R,
                                               int scale_m, double *U, int bx, int by) {
+  static_assert(!(FLAT && AXIS == 2), "x lines are not flattened");
   extern __shared__ double lds[];
   view.load_tables(lds, AXIS);
-  const int P = blockDim.x / CW;
+  const int P = FLAT ? by : blockDim.x / CW;
   double *ra = lds + ((view.lds_doubles(AXIS) + 1) & ~1);
   double *rc = ra + 2 * P * CW;
   double *rr = rc + 2 * P * CW;
   __syncthreads();
 
   const int cl = threadIdx.x % CW, p = threadIdx.x / CW;
+  const bool act = !FLAT || p < P;
   int L, stride, base;
   bool valid;
-  if (AXIS == 2) {
+  if constexpr (FLAT) {
+    L = AXIS == 1 ? g.ny : g.nz;
+    stride = AXIS == 1 ? g.nx : g.nx * g.ny;
+    const int ncol = g.nx * (AXIS == 1 ? g.nz : g.ny);
+    const int col = bx * CW + cl;
+    valid = act && col < ncol;
+    const int cc = min(col, ncol - 1);
+    if (AXIS == 1) {
+      const int k = cc / g.nx;
+      base = k * g.ny * g.nx + (cc - k * g.nx);
+    } else {
+      base = cc;
+    }
+  } else if (AXIS == 2) {
     const int nl = g.nz * g.ny, line = bx * CW + cl;
     valid = line < nl;
     L = g.nx;
@@ -99,12 +116,14 @@ __device__ __forceinline__ void adj_line_body(Grid g, V view, int CW, const doub
   }
   double As, Cs, Rs, Ae, Ce, Re;
   segment_reduce<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
-  ra[(2 * p) * CW + cl] = As; rc[(2 * p) * CW + cl] = Cs; rr[(2 * p) * CW + cl] = Rs;
-  ra[(2 * p + 1) * CW + cl] = Ae; rc[(2 * p + 1) * CW + cl] = Ce; rr[(2 * p + 1) * CW + cl] = Re;
+  if (act) {
+    ra[(2 * p) * CW + cl] = As; rc[(2 * p) * CW + cl] = Cs; rr[(2 * p) * CW + cl] = Rs;
+    ra[(2 * p + 1) * CW + cl] = Ae; rc[(2 * p + 1) * CW + cl] = Ce; rr[(2 * p + 1) * CW + cl] = Re;
+  }
   __syncthreads();
-  lds_pcr_ends(ra, rc, rr, p, P, CW, cl);
-  const double xs = rr[(2 * p) * CW + cl], xe = rr[(2 * p + 1) * CW + cl];
+  lds_pcr_ends(ra, rc, rr, p, P, CW, cl, act);
   if (!valid) return;
+  const double xs = rr[(2 * p) * CW + cl], xe = rr[(2 * p + 1) * CW + cl];
   double x[SEG];
   x[0] = xs;
   x[SEG - 1] = xe;
@@ -426,18 +445,13 @@ __global__ __launch_bounds__(kBatchThreads) void kb_adj_line(BatchMap m, DenseVi
   int local, t;
   if (!batch_locate(m, I, local, t)) return;
   const int cw = I.cw[AXIS];
-  int bx = local, by = 0;
-  if (AXIS != 2) {
-    const int nbx = (I.nx + cw - 1) / cw;
-    bx = local % nbx;
-    by = local / nbx;
-  }
   if constexpr (AXIS == 2)
     adj_x_staged_body<SEG>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), cw, R + I.off, scale_m,
-                           U + I.off, bx);
+                           U + I.off, local);
   else
-    adj_line_body<DenseView, SEG, AXIS>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), cw,
-                                        R + I.off, scale_m, U + I.off, bx, by);
+    adj_line_body<DenseView, SEG, AXIS, true>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), cw,
+                                              R + I.off, scale_m, U + I.off, local,
+                                              batch_line_segs(AXIS == 1 ? I.ny : I.nz));
 }
 
 // f: base pointers of the concatenated fields; f.ghist: base of the

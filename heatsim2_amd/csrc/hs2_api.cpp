@@ -301,8 +301,8 @@ hs2::BatchMap batch_map(const hs2_batch *b, int kind, int rule, int j) {
 int blocks_of(const hs2::BatchInst &I, int kind) {
   switch (kind) {
     case KIND_X2: return (I.nz * I.ny + hs2::kBatchXLines - 1) / hs2::kBatchXLines;
-    case KIND_Y: return (I.nx + I.cw[1] - 1) / I.cw[1] * I.nz;
-    case KIND_Z: return (I.nx + I.cw[0] - 1) / I.cw[0] * I.ny;
+    case KIND_Y: return (I.nx * I.nz + I.cw[1] - 1) / I.cw[1];
+    case KIND_Z: return (I.nx * I.ny + I.cw[0] - 1) / I.cw[0];
     case KIND_AX: return (I.nz * I.ny + I.cw[2] - 1) / I.cw[2];
     default: return (int)(((long long)I.nz * I.ny * I.nx + 255) / 256);
   }
@@ -1068,8 +1068,8 @@ int hs2_batch_create(const hs2_batch_inst *in, int ninst, int ckpt_every, hs2_ba
     I.nz = s.nz; I.ny = s.ny; I.nx = s.nx; I.nsteps = s.nsteps;
     I.src_step = s.src_step; I.dz = s.dz;
     I.xseg = hs2::batch_x_seg(s.nx);
-    I.cw[0] = hs2::batch_line_cw(s.nz);
-    I.cw[1] = hs2::batch_line_cw(s.ny);
+    I.cw[0] = hs2::batch_flat_cw(s.nz);
+    I.cw[1] = hs2::batch_flat_cw(s.ny);
     I.cw[2] = hs2::batch_line_cw(s.nx, hs2::kBatchXThreads);
     const long long n = (long long)s.nz * s.ny * s.nx;
     I.off = off;

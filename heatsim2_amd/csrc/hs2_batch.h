@@ -26,9 +26,13 @@ constexpr int kBatchSeg = 8;       // cells per thread segment of the line kerne
 constexpr int kBatchXLines = 8;    // x-lines per block of the batched x sweep (16 lanes each)
 constexpr int kBatchAlign = 32;    // instance offsets are multiples of this many cells
 
-// Lines side by side per block of a line kernel on lines of L cells: the
-// block's 256 threads hold P = 256/cw segments of kBatchSeg cells per line,
-// the fewest that cover L (each instance sized to its own axes).
+// Segments of kBatchSeg cells per line of L cells in the y and z line
+// kernels (the fewest that cover L), and the lines a 256-thread block holds
+// side by side, cw = floor(256 / P): consecutive columns in memory order.
+__host__ __device__ inline int batch_line_segs(int L) { return (L + kBatchSeg - 1) / kBatchSeg; }
+__host__ __device__ inline int batch_flat_cw(int L) { return kBatchThreads / batch_line_segs(L); }
+// Lines per block of the adjoint x-line kernel (threads / a power-of-two
+// count of segments covering L).
 __host__ __device__ inline int batch_line_cw(int L, int threads = kBatchThreads) {
   int p = 1;
   while (p * kBatchSeg < L) p <<= 1;
@@ -44,8 +48,9 @@ struct BatchInst {
   long long ckoff;  // first value of the checkpoints (ceil(nsteps/K) fields)
   int src_step;     // step whose source field applies (-1: none)
   int xseg;         // batch_x_seg(nx)
-  int cw[3];        // batch_line_cw of the z, y, x line lengths (x: of the
-                    // adjoint x-line kernel's kBatchXThreads)
+  int cw[3];        // lines per block: batch_flat_cw of the z and y line
+                    // lengths, batch_line_cw of x's for the adjoint x-line
+                    // kernel's kBatchXThreads
   int pad;
   double dz;
 };

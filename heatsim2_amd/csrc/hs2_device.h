@@ -483,30 +483,36 @@ __device__ __forceinline__ double coupling(double As, double Cs, double Ae, doub
 // skipped.  The block-wide decision rides on the level's own barrier: each
 // wave posts its ballot before it, every thread reads all posts after it.
 __device__ __forceinline__ void lds_pcr_ends(double *ra, double *rc, double *rr, int p, int P,
-                                             int CW, int cl) {
+                                             int CW, int cl, bool act = true) {
+  // act: this thread holds segment p < P (threads past the block's P x CW
+  // segments only take part in the barriers)
   __shared__ int wave_more[kMaxWaves];
   const int R = 2 * P;
   for (int d = 1; d < R; d <<= 1) {
     double na[2], nc[2], nr[2];
+    if (act) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int row = 2 * p + h;
-      double aa = ra[row * CW + cl], cc = rc[row * CW + cl], r0 = rr[row * CW + cl];
-      const int rl = row - d >= 0 ? row - d : row, rh = row + d < R ? row + d : row;
-      const double m0 = row - d >= 0 ? 1.0 : 0.0, m1 = row + d < R ? 1.0 : 0.0;
-      const double aL = m0 * ra[rl * CW + cl], cL = m0 * rc[rl * CW + cl], rL = m0 * rr[rl * CW + cl];
-      const double aR = m1 * ra[rh * CW + cl], cR = m1 * rc[rh * CW + cl], rR = m1 * rr[rh * CW + cl];
-      pcr_combine(aa, cc, r0, aL, cL, rL, aR, cR, rR);
-      na[h] = aa; nc[h] = cc; nr[h] = r0;
+      for (int h = 0; h < 2; h++) {
+        const int row = 2 * p + h;
+        double aa = ra[row * CW + cl], cc = rc[row * CW + cl], r0 = rr[row * CW + cl];
+        const int rl = row - d >= 0 ? row - d : row, rh = row + d < R ? row + d : row;
+        const double m0 = row - d >= 0 ? 1.0 : 0.0, m1 = row + d < R ? 1.0 : 0.0;
+        const double aL = m0 * ra[rl * CW + cl], cL = m0 * rc[rl * CW + cl], rL = m0 * rr[rl * CW + cl];
+        const double aR = m1 * ra[rh * CW + cl], cR = m1 * rc[rh * CW + cl], rR = m1 * rr[rh * CW + cl];
+        pcr_combine(aa, cc, r0, aL, cL, rL, aR, cR, rR);
+        na[h] = aa; nc[h] = cc; nr[h] = r0;
+      }
     }
     __syncthreads();
+    if (act) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int row = 2 * p + h;
-      ra[row * CW + cl] = na[h]; rc[row * CW + cl] = nc[h]; rr[row * CW + cl] = nr[h];
+      for (int h = 0; h < 2; h++) {
+        const int row = 2 * p + h;
+        ra[row * CW + cl] = na[h]; rc[row * CW + cl] = nc[h]; rr[row * CW + cl] = nr[h];
+      }
     }
     if (d == 1) {
-      const bool more = coupling(na[0], nc[0], na[1], nc[1]) > kNegligible;
+      const bool more = act && coupling(na[0], nc[0], na[1], nc[1]) > kNegligible;
       const unsigned long long bal = __ballot(more);
       if ((threadIdx.x & (kWave - 1)) == 0) wave_more[threadIdx.x / kWave] = bal != 0ull;
     }

@@ -710,14 +710,20 @@ __global__ __launch_bounds__(G *LPB) void k_sweep_x2(Grid g, V view,
 // line stride nx*ny.  Solves (M - La/2) X = M Y - (La/2) T^n along each line.
 // X may alias Y or T^n: a block owns whole lines, and all of its loads
 // happen before its first barrier and all of its stores after its last.
-template <class V, int SEG, int AXIS, bool FULL = false, bool RY = false>
+//
+// FLAT (batched instances): block bx covers CW consecutive columns of all the
+// instance's lines taken in memory order (axis 0: the (j, i) plane, axis 1:
+// the (k, i) columns), and by is P, the segments per line; P x CW may fall
+// short of the block, whose remaining threads only take part in barriers.
+template <class V, int SEG, int AXIS, bool FULL = false, bool RY = false, bool FLAT = false>
 __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const double *Tn,
                                                    const double *Y, double *X, double *surf,
                                                    double dz, int bx, int by) {
   constexpr int axis = AXIS;
+  static_assert(!(FLAT && FULL), "FLAT lines are masked");
   extern __shared__ double lds[];
   view.load_tables(lds, axis);
-  const int P = blockDim.x / CW;
+  const int P = FLAT ? by : blockDim.x / CW;
   double *ra = lds + ((view.lds_doubles(axis) + 1) & ~1);
   double *rc = ra + 2 * P * CW;
   double *rr = rc + 2 * P * CW;
@@ -725,13 +731,28 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
 
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int cl = threadIdx.x % CW, p = threadIdx.x / CW;
-  const int i = bx * CW + cl;
-  const bool valid = i < nx;
-  const int ic = valid ? i : nx - 1;
-  const int other = by;
+  const bool act = !FLAT || p < P;
   const int L = axis == 1 ? ny : nz;
   const int stride = axis == 1 ? nx : nx * ny;
-  const int base = axis == 1 ? other * ny * nx + ic : other * nx + ic;
+  bool valid;
+  int base;
+  if constexpr (FLAT) {
+    const int ncol = nx * (axis == 1 ? nz : ny);
+    const int col = bx * CW + cl;
+    valid = act && col < ncol;
+    const int cc = min(col, ncol - 1);
+    if (axis == 1) {
+      const int k = cc / nx;
+      base = k * ny * nx + (cc - k * nx);
+    } else {
+      base = cc;
+    }
+  } else {
+    const int i = bx * CW + cl;
+    valid = i < nx;
+    const int ic = valid ? i : nx - 1;
+    base = axis == 1 ? by * ny * nx + ic : by * nx + ic;
+  }
   const int t0 = p * SEG;
 
   // ---- loads: T^n over the segment plus one halo cell each side, Y, entries
@@ -800,12 +821,14 @@ __device__ __forceinline__ void sweep_strided_body(Grid g, V view, int CW, const
     segment_reduce_halves<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re, hj);
   else
     segment_reduce<SEG>(a, c, r, As, Cs, Rs, Ae, Ce, Re);
-  ra[(2 * p) * CW + cl] = As; rc[(2 * p) * CW + cl] = Cs; rr[(2 * p) * CW + cl] = Rs;
-  ra[(2 * p + 1) * CW + cl] = Ae; rc[(2 * p + 1) * CW + cl] = Ce; rr[(2 * p + 1) * CW + cl] = Re;
+  if (act) {
+    ra[(2 * p) * CW + cl] = As; rc[(2 * p) * CW + cl] = Cs; rr[(2 * p) * CW + cl] = Rs;
+    ra[(2 * p + 1) * CW + cl] = Ae; rc[(2 * p + 1) * CW + cl] = Ce; rr[(2 * p + 1) * CW + cl] = Re;
+  }
   __syncthreads();
-  lds_pcr_ends(ra, rc, rr, p, P, CW, cl);
-  const double xs = rr[(2 * p) * CW + cl], xe = rr[(2 * p + 1) * CW + cl];
+  lds_pcr_ends(ra, rc, rr, p, P, CW, cl, act);
   if (!valid) return;
+  const double xs = rr[(2 * p) * CW + cl], xe = rr[(2 * p + 1) * CW + cl];
   double x[SEG];
   x[0] = xs;
   x[SEG - 1] = xe;
@@ -1277,12 +1300,10 @@ __global__ __launch_bounds__(kBatchThreads) void kb_sweep_strided(
   BatchInst I;
   int local, t;
   if (!batch_locate(m, I, local, t)) return;
-  const int cw = I.cw[AXIS];
-  const int nbx = (I.nx + cw - 1) / cw;
   double *surf = (AXIS == 0 && hist) ? hist + I.hoff + (long long)t * I.ny * I.nx : nullptr;
-  sweep_strided_body<DenseView, SEG, AXIS>(Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), cw,
-                                           Tn + I.off, Y + I.off, X + I.off, surf, I.dz,
-                                           local % nbx, local / nbx);
+  sweep_strided_body<DenseView, SEG, AXIS, false, false, true>(
+      Grid{I.nz, I.ny, I.nx}, offset_view(v, I.off), I.cw[AXIS], Tn + I.off, Y + I.off,
+      X + I.off, surf, I.dz, local, batch_line_segs(AXIS == 1 ? I.ny : I.nz));
 }
 
 hipError_t launch_batch_x2(const BatchMap &m, int nblocks, const DenseView &v, const double *Tn,
