@@ -77,14 +77,32 @@ def source_tables(ADI_params, t, dt, volumetric, volumetric_elements):
     return (vtab if active else None), sfield
 
 
+def _same_map(cached, vmap):
+    """Whether vmap holds the cached map's bytes (compared eight at a time)."""
+    if cached.shape != vmap.shape:
+        return False
+    a, b = cached.reshape(-1), np.ascontiguousarray(vmap).reshape(-1)
+    if a.size % 8 == 0:
+        a, b = a.view(np.uint64), b.view(np.uint64)
+    return np.array_equal(a, b)
+
+
 def _device_vmap(ADI_params, volumetric_elements):
+    """The volumetric map on the device, uploaded again only when its
+    contents change.  A read-only array that is the very object of the
+    previous call is taken as unchanged without a comparison (so a caller can
+    skip the per-call host compare with arr.setflags(write=False))."""
     cached = getattr(ADI_params, "_vmap_host", None)
-    if cached is None or cached.shape != volumetric_elements.shape or \
-            not np.array_equal(cached, volumetric_elements):
+    same_ro = (cached is not None and
+               getattr(ADI_params, "_vmap_src", None) is volumetric_elements and
+               isinstance(volumetric_elements, np.ndarray) and
+               not volumetric_elements.flags.writeable)
+    if not same_ro and (cached is None or not _same_map(cached, volumetric_elements)):
         if volumetric_elements.dtype != np.uint8:
             raise ValueError("volumetric_elements must be uint8")
         ADI_params._vmap_host = np.array(volumetric_elements, copy=True)
         ADI_params._vmap_dev = torch.from_numpy(ADI_params._vmap_host).to(_lib.device())
+    ADI_params._vmap_src = volumetric_elements
     return ADI_params._vmap_dev
 
 

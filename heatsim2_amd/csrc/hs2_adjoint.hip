@@ -137,12 +137,13 @@ __device__ __forceinline__ void adj_line_body(Grid g, V view, int CW, const doub
 }
 
 // adj_line_body for x lines of per-voxel coefficients, staged through LDS:
-// the block's CW x-lines are consecutive in memory, so the right-hand side and
-// the x entries (M, g, w) are loaded cell by cell along the lines (coalesced)
-// into padded LDS rows; each thread then builds its segment's rows from LDS,
-// and the solution leaves through LDS the same way.  LDS: 6 P CW doubles of
-// reduced rows, then four arrays of CW rows of (L | 1) cells (R, M, g as
-// doubles, w as bytes).
+// the block's CW x-lines are consecutive in memory, so the right-hand side is
+// loaded cell by cell along the lines (coalesced) into padded LDS rows, and
+// the solution leaves through LDS the same way; each thread reads its
+// segment's x entries (M, g, w) from memory itself (staging them as well
+// quadrupled the block's LDS and cost more in occupancy than the strided
+// loads do).  LDS: 6 P CW doubles of reduced rows, then CW rows of (L | 1)
+// doubles.
 template <int SEG>
 __device__ __forceinline__ void adj_x_staged_body(Grid g, DenseView view, int CW, const double *R,
                                                   int scale_m, double *U, int bx) {
@@ -153,9 +154,6 @@ __device__ __forceinline__ void adj_x_staged_body(Grid g, DenseView view, int CW
   double *rc = ra + 2 * P * CW;
   double *rr = rc + 2 * P * CW;
   double *sR = rr + 2 * P * CW;
-  double *sM = sR + CW * RW;
-  double *sG = sM + CW * RW;
-  uint8_t *sW = reinterpret_cast<uint8_t *>(sG + CW * RW);
   const int nl = g.nz * g.ny, l0 = bx * CW;
   const int nlb = min(CW, nl - l0);
   const int ncell = nlb * L;
@@ -163,25 +161,18 @@ __device__ __forceinline__ void adj_x_staged_body(Grid g, DenseView view, int CW
   // UN cells per thread per batch, all loads issued before the first use
   constexpr int UN = 8;
   for (int e0 = 0; e0 < ncell; e0 += UN * (int)blockDim.x) {
-    double vr[UN], vm[UN], vg[UN];
-    uint8_t vw[UN];
+    double vr[UN];
 #pragma unroll
     for (int u = 0; u < UN; u++) {
       const long long c = c0 + min(e0 + u * (int)blockDim.x + (int)threadIdx.x, ncell - 1);
       vr[u] = R[c];
-      vm[u] = view.M[c];
-      vg[u] = view.g[2][c];
-      vw[u] = view.w[c];
     }
 #pragma unroll
     for (int u = 0; u < UN; u++) {
       const int e = e0 + u * (int)blockDim.x + (int)threadIdx.x;
       if (e < ncell) {
-        const int lc = e / L, o = lc * RW + (e - lc * L);
-        sR[o] = vr[u];
-        sM[o] = vm[u];
-        sG[o] = vg[u];
-        sW[o] = vw[u];
+        const int lc = e / L;
+        sR[lc * RW + (e - lc * L)] = vr[u];
       }
     }
   }
@@ -189,10 +180,11 @@ __device__ __forceinline__ void adj_x_staged_body(Grid g, DenseView view, int CW
   const int cl = threadIdx.x % CW, p = threadIdx.x / CW;
   const bool valid = cl < nlb;
   const int row = (valid ? cl : 0) * RW;
+  const long long grow = c0 + (long long)(valid ? cl : 0) * L;
   const int t0 = p * SEG;
   auto entry = [&](int t) {
-    const int o = row + min(max(t, 0), L - 1);
-    return Entry{sM[o], (double)sW[o], sG[o]};
+    const long long o = grow + min(max(t, 0), L - 1);
+    return Entry{view.M[o], (double)view.w[o], view.g[2][o]};
   };
   double a[SEG], c[SEG], r[SEG];
   Entry em = entry(t0 - 1);
@@ -505,12 +497,11 @@ __global__ void kb_counter(int *ctr, int which, int value, int add) {
 
 hipError_t launch_batch_adj_line(const BatchMap &m, int nblocks, int axis, const DenseView &v,
                                  const double *R, int scale_m, double *U, hipStream_t s) {
-  // x lines: blocks of kBatchXThreads staging CW rows of (L | 1) cells,
-  // CW L <= kBatchXThreads kBatchSeg
+  // x lines: blocks of kBatchXThreads staging CW rows of (L | 1) right-hand
+  // sides, CW L <= kBatchXThreads kBatchSeg
   const int threads = axis == 2 ? kBatchXThreads : kBatchThreads;
   const size_t rows = (size_t)kBatchXThreads * kBatchSeg + kBatchXThreads;
-  const size_t shm = sizeof(double) * 6 * (size_t)threads +
-                     (axis == 2 ? (3 * sizeof(double) + 1) * rows : 0);
+  const size_t shm = sizeof(double) * (6 * (size_t)threads + (axis == 2 ? rows : 0));
   const dim3 grid(nblocks), block(threads);
   switch (axis) {
     case 0: hipLaunchKernelGGL((kb_adj_line<kBatchSeg, 0>), grid, block, shm, s, m, v, R, scale_m, U); break;

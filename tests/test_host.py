@@ -390,3 +390,19 @@ def test_rocprof_summary_attribution_and_committed_traffic():
     # the profiled configuration hands y its right-hand side (16 B per cell)
     for k, algo in (("sweep_x", 16), ("sweep_y", 16), ("sweep_z", 24)):
         assert algo * 256 ** 3 <= tr["traffic_bytes"][k] < 2 * algo * 256 ** 3
+
+
+def test_volumetric_map_change_detection():
+    """The source map is uploaded again exactly when a byte changes (eight
+    bytes compared at a time, odd sizes byte by byte)."""
+    from heatsim2_amd.alternatingdirection import _same_map
+    rng = np.random.default_rng(3)
+    for shape in ((4, 6, 8), (3, 5, 7)):
+        a = rng.integers(0, 4, size=shape).astype(np.uint8)
+        assert _same_map(a, a.copy())
+        for flat in (0, a.size // 2, a.size - 1):
+            b = a.copy()
+            b.reshape(-1)[flat] ^= 1
+            assert not _same_map(a, b)
+        assert not _same_map(a, a[:-1])
+        assert _same_map(a, np.asfortranarray(a))
