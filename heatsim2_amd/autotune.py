@@ -25,8 +25,8 @@ from heatsim2_amd import _lib
 X_COMBOS = ((1, 16, 0), (2, 16, 0), (4, 16, 0), (8, 16, 0), (16, 16, 0), (2, 32, 0), (4, 32, 0),
             (8, 32, 0), (16, 32, 0), (2, 64, 0), (4, 64, 0), (8, 64, 0), (16, 64, 0),
             (1, 16, 8), (2, 16, 8), (4, 16, 8), (8, 16, 8), (16, 16, 8), (4, 16, 16),
-            (8, 16, 16), (16, 16, 4), (16, 16, 16), (8, 32, 4), (16, 32, 4), (16, 32, 2),
-            (16, 64, 2), (8, 64, 2))
+            (8, 16, 16), (16, 16, 4), (16, 16, 16), (8, 32, 4), (16, 32, 4), (16, 32, 8),
+            (16, 32, 2), (16, 64, 2), (8, 64, 2), (16, 64, 4))
 SEGS = (2, 4, 8, 16, 32)
 LINES = (8, 16, 32, 64)
 MAX_BLOCK = 512

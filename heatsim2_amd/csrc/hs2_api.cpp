@@ -133,7 +133,8 @@ bool x_ok(int nx, hs2::XConfig c) {
       {1, 16, 0}, {2, 16, 0}, {4, 16, 0}, {8, 16, 0}, {16, 16, 0}, {2, 32, 0}, {4, 32, 0},
       {8, 32, 0}, {16, 32, 0}, {2, 64, 0}, {4, 64, 0}, {8, 64, 0}, {16, 64, 0},
       {1, 16, 8}, {2, 16, 8}, {4, 16, 8}, {8, 16, 8}, {16, 16, 8}, {4, 16, 16}, {8, 16, 16},
-      {16, 16, 4}, {16, 16, 16}, {8, 32, 4}, {16, 32, 4}, {16, 32, 2}, {16, 64, 2}, {8, 64, 2}};
+      {16, 16, 4}, {16, 16, 16}, {8, 32, 4}, {16, 32, 4}, {16, 32, 8}, {16, 32, 2}, {16, 64, 2},
+      {8, 64, 2}, {16, 64, 4}};
   for (auto &cb : combos)
     if (cb[0] == c.cpl && cb[1] == c.g && cb[2] == c.lpb) return c.cpl * c.g >= nx;
   return false;

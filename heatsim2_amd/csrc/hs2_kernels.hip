@@ -208,9 +208,9 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
       // in one phase-1 batch (lines l and l + W, l = 2 W batch + W b + slab)
       // are rows j and j + 1 of one plane (each is the other's y neighbour),
       // and its next batch holds the same two rows one plane up
-      // W = lines per quad slab (NT threads over lines of SEG*G/4 quads):
-      // 4 for 16 x 16-cell lines in 16-line blocks, 1 for 16 x 32-cell lines
-      // in 4-line blocks; a tile is 2 W planes x 2 rows
+      // W = lines per quad slab (NT threads over lines of SEG*G/4 quads) =
+      // LPB/4 for 16-cell lanes (4 for 16-line blocks, 1 for 4-line ones); a
+      // tile is 2 W planes x 2 rows
       constexpr int W = NT / (SEG * G / 4);
       const int nbj = (ny + 1) / 2;
       const int kt = blk / nbj, jt = blk - kt * nbj;
@@ -259,9 +259,8 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
   // T^n.  The batches are unrolled to keep those in registers.
   constexpr int NIT = FQ ? SEG / (4 * UQ) : 1;
   static_assert(!RY || (FQ && BUF), "RY: full-line class-map kernel");
-  static_assert(!PZ || (FQ && BUF && UQ == 2 && SEG == 16 &&
-                        ((G == 16 && LPB == 16) || (G == 32 && LPB == 4))),
-                "PZ: 16 x 16-cell lines in 16-line blocks or 16 x 32 in 4-line blocks");
+  static_assert(!PZ || (FQ && BUF && UQ == 2 && SEG == 16 && LPB % 4 == 0),
+                "PZ: 16-cell lanes, blocks of 4, 8 or 16 lines (tiles of LPB/2 planes x 2 rows)");
   [[maybe_unused]] double hly[RY ? NIT : 1][UQ][4];
   [[maybe_unused]] uint32_t rxw[RY ? NIT : 1][UQ];
   // PZ with RY: a thread's four quads are rows j, j+1 of planes k, k+1 over
@@ -957,11 +956,10 @@ static hipError_t launch_x2_t(const Grid &g, const V &v, int ty, const double *T
       (std::is_same<V, DenseView>::value ? (2 * sizeof(double) + 1) * (size_t)LPB * RS : 0);
   const bool full = x2_full(g, SEG * G, LPB, ty);
   if constexpr (std::is_same<V, ClassView>::value && SEG % 4 == 0) {
-    // 16-cell lanes, 16 x 16 in 8 x 2 tiles or 32 x 16 in 2 x 2 tiles: the
-    // paired line order (source-free
-    // steps on full lines always take this R_y path: x_ry_ok holds exactly
-    // when the lines are full)
-    constexpr bool kPZ = SEG == 16 && ((G == 16 && LPB == 16) || (G == 32 && LPB == 4));
+    // 16-cell lanes in blocks of 4, 8 or 16 lines as tiles of LPB/2 planes x
+    // 2 rows: the paired line order (source-free steps on full lines always
+    // take this R_y path: x_ry_ok holds exactly when the lines are full)
+    constexpr bool kPZ = SEG == 16 && LPB % 4 == 0;
     if (ry) {  // the caller checked x_ry_ok and that the step has no sources
       if (!full || src) return hipErrorInvalidValue;
       if (kPZ && ty == 2)
@@ -1019,7 +1017,8 @@ static hipError_t launch_x_v(XConfig cfg, const Grid &g, const V &v, const doubl
     return launch_x2_t<V, C, GG, LL>(g, v, cfg.ty, Tn, Ts, vmap, vtab, sfield, ry, s);
   HS2_X2(1, 16, 8) HS2_X2(2, 16, 8) HS2_X2(4, 16, 8) HS2_X2(8, 16, 8) HS2_X2(16, 16, 8)
   HS2_X2(4, 16, 16) HS2_X2(8, 16, 16) HS2_X2(16, 16, 4) HS2_X2(16, 16, 16)
-  HS2_X2(8, 32, 4) HS2_X2(16, 32, 4) HS2_X2(16, 32, 2) HS2_X2(16, 64, 2) HS2_X2(8, 64, 2)
+  HS2_X2(8, 32, 4) HS2_X2(16, 32, 4) HS2_X2(16, 32, 8) HS2_X2(16, 32, 2) HS2_X2(16, 64, 2)
+  HS2_X2(8, 64, 2) HS2_X2(16, 64, 4)
 #undef HS2_X2
   return hipErrorInvalidValue;
 }
@@ -1036,7 +1035,7 @@ static hipError_t launch_x2c_t(const Grid &g, const CombView &v, int ty, const d
   const size_t shm = sizeof(double) * (((v.lds_doubles(-1) + 1) & ~1) + LPB * RS) +
                      (size_t)LPB * SEG * G;
   if (!x2_full(g, SEG * G, LPB, ty)) return hipErrorInvalidValue;
-  constexpr bool kPZ = SEG == 16 && ((G == 16 && LPB == 16) || (G == 32 && LPB == 4));
+  constexpr bool kPZ = SEG == 16 && LPB % 4 == 0;
   const uint8_t *vmap = nullptr;
   const double *vtab = nullptr, *sfield = nullptr;
   if (ry && kPZ && ty == 2)
@@ -1057,8 +1056,8 @@ hipError_t launch_sweep_x(XConfig cfg, const Grid &g, const CombView &v, const d
   if (cfg.lpb == LL && cfg.cpl == C && cfg.g == GG)                                \
     return launch_x2c_t<C, GG, LL>(g, v, cfg.ty, Tn, Ts, ry, s);
   HS2_XC(4, 16, 8) HS2_XC(8, 16, 8) HS2_XC(16, 16, 8) HS2_XC(4, 16, 16) HS2_XC(8, 16, 16)
-  HS2_XC(16, 16, 4) HS2_XC(16, 16, 16) HS2_XC(8, 32, 4) HS2_XC(16, 32, 4) HS2_XC(16, 32, 2)
-  HS2_XC(16, 64, 2) HS2_XC(8, 64, 2)
+  HS2_XC(16, 16, 4) HS2_XC(16, 16, 16) HS2_XC(8, 32, 4) HS2_XC(16, 32, 4) HS2_XC(16, 32, 8)
+  HS2_XC(16, 32, 2) HS2_XC(16, 64, 2) HS2_XC(8, 64, 2) HS2_XC(16, 64, 4)
 #undef HS2_XC
   return hipErrorInvalidValue;
 }

@@ -249,12 +249,14 @@ def test_y_from_x_path_matches_oracle(hs):
             assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, (x, n)
 
 
-@pytest.mark.parametrize("shape,x", [((16, 10, 256), (16, 16, 16)), ((4, 6, 512), (16, 32, 4))],
-                         ids=["256", "512"])
+@pytest.mark.parametrize("shape,x", [((16, 10, 256), (16, 16, 16)), ((16, 10, 256), (16, 16, 8)),
+                                     ((4, 6, 512), (16, 32, 4)), ((8, 6, 512), (16, 32, 8)),
+                                     ((4, 6, 1024), (16, 64, 4))],
+                         ids=["256", "256-8", "512", "512-8", "1024-4"])
 def test_x_plane_pairs_match_oracle(hs, shape, x):
-    """x lines of 16 lanes x 16 cells in 8 x 2 tiles, or 32 lanes x 16 cells in
-    2 x 2 tiles, run with the lines ordered so that each thread's quads are
-    neighbours in y and z and share those rows
+    """x lines of 16-cell lanes in blocks of 4, 8 or 16 lines, tiled as
+    LPB/2 planes x 2 rows, run with the lines ordered so that each thread's
+    quads are neighbours in y and z and share those rows
     (source-free steps 4-5; steps 0-3 carry sources and run the source
     kernel).  Against the oracle step by step, and bit for bit against
     1-row tiles, which take the same arithmetic per cell."""
@@ -285,8 +287,9 @@ def test_x_plane_pairs_match_oracle(hs, shape, x):
     (lambda: cases.case_laminate(16, 10, 256, nsteps=6, patch_min=2, patch_max=8), (16, 16, 16, 2)),
     (lambda: cases.case_laminate(16, 10, 256, nsteps=6, patch_min=2, patch_max=8), (16, 16, 16, 0)),
     (lambda: cases.case_two_layer(nz=4, ny=6, nx=512, nsteps=6, steel_layers=2), (16, 32, 4, 2)),
+    (lambda: cases.case_two_layer(nz=8, ny=6, nx=512, nsteps=6, steel_layers=3), (16, 32, 8, 2)),
     (lambda: cases.case_two_layer(nz=6, ny=8, nx=128, nsteps=6, steel_layers=3), (8, 16, 8, 0))],
-    ids=["256-pairs", "256-rows", "512-pairs", "128"])
+    ids=["256-pairs", "256-rows", "512-pairs", "512-pairs8", "128"])
 def test_x_combined_class_plane_matches_three_planes(hs, maker, x):
     """Source-free steps on full x lines read one combined class byte per
     cell (its z, y, x class triple) instead of three planes: bit for bit the
