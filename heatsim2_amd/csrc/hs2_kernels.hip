@@ -333,17 +333,26 @@ __device__ __forceinline__ void sweep_x2_body(Grid g, V view, const double *__re
           unpack4(wx, rx[b]);
           if constexpr (RY) rxw[it][b] = wx;
           cbuf[li[b] * (SEG * G / 4) + xq[b]] = wx;
-          if constexpr (FQ && SEG * G == 4 * kWave && NT % kWave == 0) {
-            // a wave holds one whole line, quad x0/4 in lane x0/4: the cells
-            // either side of the quad are in the neighbouring lanes (at the
-            // line's ends the cell itself, as the clamped loads give)
-            const int lane = xq[b];
+          if constexpr (FQ && (SEG * G) % (4 * kWave) == 0 && NT % kWave == 0) {
+            // a wave holds 64 consecutive quads of one line, quad x0/4 in lane
+            // x0/4 % 64: the cells either side of the quad are in the
+            // neighbouring lanes; at a wave's edge inside the line the lane
+            // loads them, at the line's ends they are the cell itself, as
+            // the clamped loads give
+            const int lane = xq[b] & (kWave - 1);
             const double l3 = __shfl_up(T[b][3], 1, kWave);
             const double r0 = __shfl_down(T[b][0], 1, kWave);
             const uint32_t rw = __shfl_down(wx, 1, kWave);
             TL[b] = lane > 0 ? l3 : T[b][0];
             TR[b] = lane < kWave - 1 ? r0 : T[b][3];
             rxn[b] = lane < kWave - 1 ? (rw & 0xffu) : rx[b][3];
+            if constexpr (SEG * G > 4 * kWave) {
+              if (lane == 0 && x0 > 0) TL[b] = buf_load_d(bT, (unsigned)(b0 + x0 - 1) * 8u);
+              if (lane == kWave - 1 && x0 + 4 < nx) {
+                TR[b] = buf_load_d(bT, (unsigned)(b0 + x0 + 4) * 8u);
+                rxn[b] = buf_load_u8(bC[2], (unsigned)(b0 + x0 + 4));
+              }
+            }
           } else {
             TL[b] = buf_load_d(bT, (unsigned)(b0 + xl) * 8u);
             TR[b] = buf_load_d(bT, (unsigned)(b0 + xr) * 8u);
