@@ -79,13 +79,15 @@ def _loss(env, p):
     return ((_history(env, p) - env["obs"]) ** 2).sum()
 
 
-def _richardson(hist_at, obs, x0, rel):
+def _richardson(hist_at, obs, x0, rel, h=None):
     """(4 D(h) - D(2h)) / 3 of the loss, D(h) = (L(x0+h) - L(x0-h)) / 2h with
-    L(a) - L(b) = sum((h_a - h_b) (h_a + h_b - 2 obs))."""
+    L(a) - L(b) = sum((h_a - h_b) (h_a + h_b - 2 obs)); h = rel |x0| unless
+    given."""
     def D(h):
         a, b = hist_at(x0 + h), hist_at(x0 - h)
         return float(((a - b) * (a + b - 2.0 * obs)).sum()) / (2 * h)
-    h = rel * abs(x0)
+    if h is None:
+        h = rel * abs(x0)
     return (4 * D(h) - D(2 * h)) / 3
 
 
@@ -138,3 +140,38 @@ def test_laminate_gradients_match_central_differences():
     ok, worst = _grad_ok([float(p["E"].grad)], [fdE])
     print("d/dE: adjoint %.9e, fd %.9e" % (float(p["E"].grad), fdE), flush=True)
     assert ok, (float(p["E"].grad), fdE, worst)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_laminate_lateral_gradients_along_smooth_directions():
+    """The lateral conductivities' gradients over every voxel at once: g . v
+    of the adjoint's dJ/dkx (and dJ/dky) against the Richardson-extrapolated
+    central difference of the loss along v, a smooth random field
+    (Gaussian-filtered noise, sigma = 4 cells, seed 6) scaled to 32 % of the
+    conductivity at its largest, with steps of 1 and 2 v.  Along v the
+    history moves by ~1e-8 K per value, so the differences' own round-off
+    limits them to about 2e-6: Richardson estimates from steps of 4 % to 64 %
+    agree with each other, and with g . v, to that (measured: 4.9e-7 for kx,
+    1.2e-6 for ky at these steps).  Bar: |g . v - fd| <= 5e-6 |fd|."""
+    from scipy.ndimage import gaussian_filter
+    dev = torch.device("cuda")
+    env, base = _setup(dev)
+    p = {k: v.clone().requires_grad_(True) for k, v in base.items()}
+    _loss(env, p).backward()
+    rng = np.random.default_rng(6)
+    for name in ("kx", "ky"):
+        v = gaussian_filter(rng.standard_normal(env["shape"]), sigma=4.0)
+        v = torch.tensor(v / np.abs(v).max(), dtype=torch.float64, device=dev)
+        v = v * 0.32 * float(base[name].abs().max())
+        gv = float((p[name].grad * v).sum())
+
+        def hist_at(eps, name=name, v=v):
+            q = {k: t.detach().clone() for k, t in p.items()}
+            q[name] = q[name] + eps * v
+            with torch.no_grad():
+                return _history(env, q)
+        fd = _richardson(hist_at, env["obs"], 0.0, None, h=1.0)
+        print("d/d%s along v: adjoint %.12e, fd %.12e, rel %.2e"
+              % (name, gv, fd, abs(gv - fd) / abs(fd)), flush=True)
+        assert abs(gv - fd) <= 5e-6 * abs(fd), (name, gv, fd)
