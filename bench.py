@@ -31,6 +31,11 @@ device-to-device copy of one field on the same box (read + write bytes per
 copy, events on torch's stream); step_frac_of_copy puts the step's canonical
 64 B per cell-step against it.
 
+single_gradient: one gradient evaluation of the single-instance inversion
+workload (64x128x128 laminate, 500 steps, checkpoints every 50).
+batched: gradients/s of 256 uneven instances through the batched forward +
+adjoint against the same instances one after another.
+
 cpu_baseline: the oracle's reference-structured step (assembled permuted
 CSR operators, global Thomas with precomputed LU, transposes -- the
 upstream's structure) on one host thread, over a few steps of the same
@@ -153,6 +158,65 @@ def cpu_baseline(case, nsteps, laminate_steps):
                        % ((nsteps,) + tuple(case["shape"]) + (dt, t_setup) +
                           (float(fx["ref_run_s"]), float(fx["port_run_s"]), laminate_steps,
                            lam_dt, lam_setup) + tuple(case["shape"])))
+
+
+def single_gradient(hs, log, reps=3):
+    """The single-instance inversion workload: one gradient evaluation
+    (per-voxel coefficients, forward with checkpoints, loss, hand-written
+    adjoint, chain rule to kx, ky, kz, rho*c and the flash amplitude) of the
+    64x128x128 anisotropic laminate, 500 steps at dt = 2 ms, checkpoints every
+    50 steps, at the homogeneous guess against the history of a smooth
+    random kz field (seed 3, sigma 8 cells) plus 0.01 K noise (seed 4)."""
+    import torch
+    from scipy.ndimage import gaussian_filter
+    from hs2_cases import cases
+    from heatsim2_amd import _lib
+    dev = _lib.device()
+    nsteps, K = 500, 50
+    case = cases.case_laminate(64, 128, 128, nsteps=nsteps)
+    shape = case["shape"]
+    g = gaussian_filter(np.random.default_rng(3).standard_normal(shape), sigma=8.0)
+    g = (g - g.mean()) / g.std()
+    _, bz, by, bx, _ = case["maps"]
+    bnds = ((hs.boundary_conducting_anisotropic,), (hs.boundary_insulating,),
+            (hs.boundary_thininsulatinglayer, case["boundaries"][2][1]))
+    _fixed, kz0, ky0, kx0, rho, c, _t = case["materials"][0]
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
+    base = dict(kz=t(np.full(shape, kz0)), ky=t(np.full(shape, ky0)), kx=t(np.full(shape, kx0)),
+                rhoc=t(np.full(shape, rho * c)), E=t(1e4))
+    mask = torch.zeros(shape, dtype=torch.float64, device=dev)
+    mask[0] = 1.0
+    plan = hs.adjoint.GradPlan(shape, case["d"], case["dt"])
+    T0 = torch.zeros(shape, dtype=torch.float64, device=dev)
+
+    def history(p):
+        coef = hs.adjoint.per_voxel_coefficients(p["kz"], p["ky"], p["kx"], p["rhoc"],
+                                                 case["dt"], case["d"], bnds, bz, by, bx)
+        S0 = p["E"] * mask / (case["d"][0] * case["dt"])
+        return hs.adjoint.surface_history(plan, coef, T0, nsteps, sources=[(0, S0)],
+                                          ckpt_every=K)
+    with torch.no_grad():
+        h = history(dict(base, kz=t(kz0 * (1.0 + 0.3 * g))))
+        obs = h + 0.01 * t(np.random.default_rng(4).standard_normal(tuple(h.shape)))
+    times = []
+    for rep in range(reps + 1):  # the first evaluation warms the code objects
+        p = {k: v.clone().requires_grad_(True) for k, v in base.items()}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss = ((history(p) - obs) ** 2).sum()
+        loss.backward()
+        torch.cuda.synchronize()
+        if rep:
+            times.append(time.perf_counter() - t0)
+    sec = float(np.median(times))
+    cells = int(np.prod(shape))
+    log("single-instance gradient (64x128x128 laminate, 500 steps, K=50): %.3f s per evaluation"
+        % sec)
+    return dict(workload="64x128x128 anisotropic laminate, 500 steps at dt = 2 ms, checkpoints "
+                         "every 50 steps; gradients w.r.t. per-voxel kx, ky, kz, rho*c and the "
+                         "flash amplitude of an L2 surface-history misfit, FP64",
+                seconds=sec, gradients_per_s=1.0 / sec,
+                cell_steps_per_s=cells * nsteps / sec, evaluations_timed=reps)
 
 
 def strongly_coupled(hs, size, steps, log):
@@ -604,12 +668,15 @@ def main():
         out["strongly_coupled"]["slowdown_vs_value"] = round(value / out["strongly_coupled"]["value"], 4)
         torch.cuda.empty_cache()
     if not args.no_batch:
+        log("single-instance gradient ...")
+        out["single_gradient"] = single_gradient(hs, log)
+        torch.cuda.empty_cache()
         log("batched gradients ...")
         out["batched"] = batched_gradients(hs, args.batch_instances, args.batch_seed,
                                            args.ckpt_every, args.plain_sample, log)
         torch.cuda.empty_cache()
     else:
-        out["batched"] = None
+        out["batched"] = out["single_gradient"] = None
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps, args.cpu_laminate_steps)
