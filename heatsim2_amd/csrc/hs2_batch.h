@@ -68,11 +68,12 @@ struct BatchMap {
 // Logical block of hardware block b among nb.  Hardware hands block b to XCD
 // b % 8, each with its own L2; consecutive logical blocks touch neighbouring
 // rows (and share cache lines of rows that do not start on a line), so they
-// go to one XCD in chunks of kBatchChunk, and the chunks go round the XCDs so
+// go to one XCD in chunks of kBatchChunk (32: measured best among 1 to
+// 1024), and the chunks go round the XCDs so
 // that every XCD keeps a share of every instance (instances finish at
 // different steps).  Blocks past the last whole round of chunks keep their
 // index.
-constexpr int kBatchChunk = 128;
+constexpr int kBatchChunk = 32;
 __device__ __forceinline__ int batch_block(int b, int nb) {
   constexpr int R = 8 * kBatchChunk;
   if (b >= nb / R * R) return b;
