@@ -75,10 +75,7 @@ struct BatchMap {
 // index.
 constexpr int kBatchChunk = 32;
 __device__ __forceinline__ int batch_block(int b, int nb) {
-  constexpr int R = 8 * kBatchChunk;
-  if (b >= nb / R * R) return b;
-  const int x = b & 7, k = b >> 3;
-  return ((k / kBatchChunk) * 8 + x) * kBatchChunk + k % kBatchChunk;
+  return xcd_remap_chunks<kBatchChunk>(b, nb);
 }
 
 // Instance, local block and step of this block; false if the instance has

@@ -211,6 +211,17 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks) {
   return x * q + (x < r ? x : r) + idx;
 }
 
+// The same in chunks: consecutive logical blocks go to one XCD C at a time,
+// and the chunks go round the XCDs (blocks past the last whole round of 8 C
+// keep their index).  A bijection on [0, nblocks).
+template <int C>
+__device__ __forceinline__ int xcd_remap_chunks(int b, int nblocks) {
+  constexpr int R = 8 * C;
+  if (b >= nblocks / R * R) return b;
+  const int x = b & 7, k = b >> 3;
+  return ((k / C) * 8 + x) * C + k % C;
+}
+
 // ------------------------------------------------------- lane exchange
 // Within groups of G lanes (G divides 64).  up(v, s): value of lane l-s of
 // the same group, 0 for the first s lanes; down(v, s): lane l+s, 0 for the

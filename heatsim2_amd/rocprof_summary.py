@@ -93,7 +93,11 @@ def sweep_of_dispatches(dispatches):
     return {s: v for s, (_, v) in sweep_kernels(dispatches).items()}
 
 
-def trace_summary(directory):
+def trace_summary(directory, skip=0, count=0):
+    """Per-kernel statistics of a kernel trace, and per sweep the mean device
+    time of its most launched instantiation: over every launch, and with
+    count > 0 also over launches skip .. skip + count - 1 in dispatch order
+    (the timed run of a bench, apart from its warm-up and timing passes)."""
     lines = []
     stats = _rows(_one(directory, "_kernel_stats.csv"))
     lines.append("%-58s %7s %11s %11s %11s" % ("kernel", "calls", "avg_us", "min_us", "max_us"))
@@ -112,6 +116,14 @@ def trace_summary(directory):
         if v:
             lines.append("  sweep_%s: %d launches, mean %.2f us, min %.2f us" % (
                 s, len(v), sum(v) / len(v), min(v)))
+    if count > 0:
+        lines.append("timed run: launches %d .. %d of each" % (skip, skip + count - 1))
+        for s in ("x", "y", "z"):
+            v = per.get(s, [])[skip:skip + count]
+            if v:
+                lines.append("  sweep_%s: %d launches, mean %.2f us, min %.2f us" % (
+                    s, len(v), sum(v) / len(v), min(v)))
+        per = {s: v[skip:skip + count] for s, v in per.items()}
     return "\n".join(lines), {s: (sum(v) / len(v) if v else None) for s, v in per.items()}
 
 
@@ -234,6 +246,8 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("trace")
     t.add_argument("dir")
+    t.add_argument("--skip", type=int, default=0, help="launches of each sweep before the timed run")
+    t.add_argument("--count", type=int, default=0, help="launches of each sweep in the timed run")
     r = sub.add_parser("traffic")
     for k in ("fetch", "write", "copy_fetch", "copy_write"):
         r.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
@@ -249,7 +263,7 @@ def main(argv=None):
     k.add_argument("--match", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "trace":
-        print(trace_summary(a.dir)[0])
+        print(trace_summary(a.dir, a.skip, a.count)[0])
     elif a.cmd == "kernels":
         print(kernels_summary(a))
     else:
