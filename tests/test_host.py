@@ -406,3 +406,18 @@ def test_volumetric_map_change_detection():
             assert not _same_map(a, b)
         assert not _same_map(a, a[:-1])
         assert _same_map(a, np.asfortranarray(a))
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without libhs2.so the package raises on first use; it never falls
+    back to a CPU path."""
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libhs2.so"))
+    with pytest.raises(ImportError, match="not built"):
+        _lib.load()
+    with pytest.raises(ImportError, match="not built"):
+        hs.setup(0.0, 0.0, 0.0, 1e-3, 1e-3, 1e-3, 2, 2, 2, 1e-3,
+                 ((hs.TEMPERATURE_COMPUTE, 1.0, 1.0, 1.0),), ((hs.boundary_insulating,),),
+                 ((hs.NO_SOURCE,),), np.zeros((2, 2, 2), np.uint8),
+                 np.zeros((3, 2, 2), np.uint8), np.zeros((2, 3, 2), np.uint8),
+                 np.zeros((2, 2, 3), np.uint8), np.zeros((2, 2, 2), np.uint8))
