@@ -421,3 +421,40 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
                  ((hs.NO_SOURCE,),), np.zeros((2, 2, 2), np.uint8),
                  np.zeros((3, 2, 2), np.uint8), np.zeros((2, 3, 2), np.uint8),
                  np.zeros((2, 2, 3), np.uint8), np.zeros((2, 2, 2), np.uint8))
+
+
+def test_trace_summary_timed_run(tmp_path):
+    """The trace summary's per-sweep means: over every launch of the most
+    launched instantiation, and over launches skip .. skip + count - 1 in
+    dispatch order (a bench's timed run, without its warm-up and timing
+    passes), on a synthetic kernel trace."""
+    import csv
+    names = {"x": "void hs2::k_sweep_x2<hs2::CombView, 16, 16, 16, false, true, true, true>(a)",
+             "y": "void hs2::k_sweep_strided<hs2::ClassView, 16, 1, true, true>(a)",
+             "z": "void hs2::k_sweep_strided<hs2::ClassView, 16, 0, true, false>(a)"}
+    dur = {"x": 70.0, "y": 56.0, "z": 74.0}
+    rows, d, t = [], 0, 0
+    for n in range(30):  # 5 warm-up, 20 timed, 5 slower timing-pass launches
+        for s in "xyz":
+            us = dur[s] + (10.0 if n >= 25 else 0.0) + (5.0 if n < 5 else 0.0)
+            rows.append({"Dispatch_Id": d, "Kernel_Name": names[s], "Start_Timestamp": t,
+                         "End_Timestamp": t + int(us * 1000)})
+            d += 1
+            t += int(us * 1000) + 1000
+    with open(tmp_path / "kt_kernel_trace.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=list(rows[0]))
+        w.writeheader()
+        w.writerows(rows)
+    with open(tmp_path / "kt_kernel_stats.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Name", "Calls", "TotalDurationNs", "AverageNs",
+                                          "MinNs", "MaxNs"])
+        w.writeheader()
+        for s in "xyz":
+            w.writerow({"Name": names[s], "Calls": 30, "TotalDurationNs": 1, "AverageNs": 1,
+                        "MinNs": 1, "MaxNs": 1})
+    text, means = rocprof_summary.trace_summary(str(tmp_path), skip=5, count=20)
+    assert means == {s: dur[s] for s in "xyz"}
+    assert "timed run: launches 5 .. 24 of each" in text
+    _, all_means = rocprof_summary.trace_summary(str(tmp_path))
+    for s in "xyz":
+        assert abs(all_means[s] - (dur[s] + (5 * 10.0 + 5 * 5.0) / 30)) < 1e-9
