@@ -458,3 +458,14 @@ def test_trace_summary_timed_run(tmp_path):
     _, all_means = rocprof_summary.trace_summary(str(tmp_path))
     for s in "xyz":
         assert abs(all_means[s] - (dur[s] + (5 * 10.0 + 5 * 5.0) / 30)) < 1e-9
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message before
+    touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
