@@ -489,3 +489,31 @@ def _every_launch_shape(hs, case):
             T = hs.run_adi_steps(P, S, n * case["dt"], case["dt"], T, args["vmap"],
                                  args["volumetric"])
             assert _rel(T.cpu().numpy(), ref[n], rise) < TOL, (axis, cand, n)
+
+
+def test_run_timed_advances_like_run(hs):
+    """hs2_run_timed times the same kernels hs2_run launches: methods 0 and 1
+    advance the field by nsteps and method 2 by 4 nsteps (its extra launches
+    write a scratch field only), each bit for bit as hs2_run; the times are
+    positive, and an unknown method is refused."""
+    import ctypes
+    from heatsim2_amd import _lib
+    case = cases.case_two_layer(nz=16, ny=32, nx=64, nsteps=1, steel_layers=4)
+    args = to_package_args(hs, case)
+    P, S = hs.setup(*args["setup"])
+    L = _lib.load()
+    T0, _h = hs.run_adi_steps_many(P, 0.0, case["dt"], 1, torch.from_numpy(case["T0"]).cuda(),
+                                   args["vmap"], args["volumetric"])
+    n = 5
+    for method, steps in ((0, n), (1, n), (2, 4 * n)):
+        A, B = T0.clone(), T0.clone()
+        _lib.check("hs2_run", L.hs2_run(P.plan, _lib.ptr(A), steps, None, None, 0, None,
+                                        _lib.stream_handle()))
+        ms = (ctypes.c_double * 4)()
+        _lib.check("hs2_run_timed", L.hs2_run_timed(P.plan, _lib.ptr(B), n, method, ms,
+                                                    _lib.stream_handle()))
+        torch.cuda.synchronize()
+        assert torch.equal(A, B), method
+        assert all(v > 0 for v in ms), (method, list(ms))
+    ms = (ctypes.c_double * 4)()
+    assert L.hs2_run_timed(P.plan, _lib.ptr(T0), n, 3, ms, _lib.stream_handle()) != 0
