@@ -27,7 +27,7 @@ pairs' sum over the step time of a run with no events between kernels,
 split in three: event_pair_overhead_ms); peak = 8 TB/s.  traffic: the HBM bytes per launch of that kernel read from
 the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
 (profiles/hbm_traffic_two_layer_256.txt, copy-calibrated).  copy_GBps: a
-device-to-device copy of one field on the same box (read + write bytes per
+device-to-device copy of one field on the same GPU in the same run (read + write bytes per
 copy, events on torch's stream); step_frac_of_copy puts the step's canonical
 64 B per cell-step against it.
 
@@ -623,7 +623,7 @@ def main():
         log("  %s: %.4f ms/launch, %.0f GB/s algorithmic (%.1f%% of 8 TB/s)"
             % (names[i], kms[i], gbs[i], 100 * gbs[i] / HBM_PEAK_GBS))
     step_gbs = CANONICAL_STEP_BYTES * value / 1e9
-    # a plain device copy of one field on the same box, as a practical
+    # a plain device copy of one field on the same GPU, as a practical
     # ceiling next to the 8 TB/s peak (read + write bytes per copy)
     copy_gbs = device_copy_gbs(T)
     log("device copy: %.0f GB/s; the step reaches %.1f%% of it on 64 B per cell-step"
